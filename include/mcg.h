@@ -1,0 +1,262 @@
+/*
+ * mcg.h -- C ABI of the MI355X phonon Monte Carlo trajectory engine.
+ *
+ * The engine replaces the per-bundle loop of the reference's field solve,
+ * FieldProblem::solve (montecarlo/problem.cpp:370-445), and everything it
+ * calls per scatter-step: free flight (Subdomain::advect, subdomain.cpp:161-192),
+ * relaxation-time scattering (Material::scatter, material.cpp:226-231),
+ * boundary reflection (the Boundary::scatter family, boundary.cpp:283-522)
+ * and grid tallying (Field::accumulate, field.cpp:92-220).
+ *
+ * Conventions
+ *  - Plain C: fixed-width integers, doubles, plain pointers and sizes.
+ *  - Every entry point returns an int status (MCG_OK = 0); on failure the
+ *    message is available from mcg_last_error() (thread-local).  No C++
+ *    exception crosses this boundary and nothing aborts on bad input.
+ *  - The caller owns every host buffer passed in or out.  A scene owns the
+ *    device memory it uploads.  One host thread drives a scene at a time.
+ *  - Matrices are 3x3, column-major (the reference stores them as Eigen
+ *    Matrix3d, which is column-major): m[0..2] is column 0.
+ *  - Lengths in metres, times in seconds, temperatures in kelvin.  Wall and
+ *    volume temperatures are deviational (relative to the material's T).
+ */
+#ifndef MCG_H
+#define MCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCG_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+enum mcg_status {
+    MCG_OK = 0,
+    MCG_EINVAL = 1,    /* invalid argument or inconsistent scene          */
+    MCG_ENOMEM = 2,    /* host or device allocation failed                */
+    MCG_EDEVICE = 3,   /* HIP runtime error or no usable GPU              */
+    MCG_ERANGE = 4,    /* output buffer too small / size beyond a limit   */
+    MCG_EAGAIN = 5,    /* serving: result not ready yet                   */
+    MCG_ENOTFOUND = 6  /* serving: unknown request id                     */
+};
+
+/* ------------------------------------------------------------- material
+ * Inputs of Material::Material(disp, relax, T) (material.cpp:82-162),
+ * already parsed from the reference's two text formats:
+ *  disp:  nw rows x (2 + 2*np) columns, row-major: omega, domega, then for
+ *         each branch p the pair (vel_p, dos_p)        (material.cpp:90-107)
+ *  relax: np rows x 8 columns, row-major: two mechanisms x (c0, c1, c2, c3),
+ *         1/tau = sum_j c0 * omega^c1 * T^c2 * exp(-c3/T), a mechanism with
+ *         c0 == 0 is skipped                            (material.cpp:113-134)
+ */
+typedef struct mcg_material_spec {
+    int64_t nw;
+    int64_t np;
+    const double* disp;
+    const double* relax;
+    double temp;
+} mcg_material_spec;
+
+/* Tables Material::Material derives (material.cpp:116-161).  Arrays are
+ * nw x np row-major and owned by the caller (any pointer may be NULL). */
+typedef struct mcg_material_tables {
+    double* tau;
+    double* vel;
+    double* energy_pdf;
+    double* flux_pdf;
+    double* scat_pdf;
+    double energy_sum;   /* Material::energySum  (heat capacity C)      */
+    double flux_sum;     /* Material::fluxSum    (sum of v*C)           */
+    double scat_sum;     /* Material::scatSum                            */
+    double cond;         /* Material::cond       (bulk k)                */
+    int64_t nmodes;      /* cells with energy_pdf > 0 (the draw support) */
+} mcg_material_tables;
+
+/* ------------------------------------------------------------- geometry */
+enum mcg_bdry_kind {
+    MCG_SPEC = 0,      /* SpecBoundary     boundary.cpp:283-287             */
+    MCG_DIFF = 1,      /* DiffBoundary     boundary.cpp:308-312             */
+    MCG_ISOT = 2,      /* IsotBoundary<S>  boundary.cpp:455-460 (emits)     */
+    MCG_PERI = 3,      /* PeriBoundary<S>  boundary.cpp:516-522 (emits)     */
+    MCG_INTER = 4,     /* InterBoundary    boundary.cpp:349-359             */
+    MCG_PARTIAL = 5    /* extension: specular with probability p, else
+                          diffuse (the p = 1 / p = 0 ends are Spec / Diff)  */
+};
+
+enum mcg_cell_kind {
+    MCG_PARALLELEPIPED = 0,   /* subdomain.h:128-198, faces Bac Lef Bot Fro Rig Top */
+    MCG_TRIPRISM = 1,         /* subdomain.h:206-277, faces Bac Lef Bot Dia Top     */
+    MCG_TETRAHEDRON = 2       /* subdomain.h:285-352, faces Bac Lef Bot Dia         */
+};
+
+/* One face of a cell: the boundary type chosen by the template argument in
+ * the reference (e.g. Parallelepiped<IsotBoundary<Parallelogram>, Spec,
+ * Spec>) and the constructor's temperature T(i). */
+typedef struct mcg_face_spec {
+    int32_t kind;          /* mcg_bdry_kind                                   */
+    int32_t reserved;
+    double temp;           /* deviational wall temperature, Isot / Peri only  */
+    double specularity;    /* MCG_PARTIAL only, in [0, 1]                     */
+} mcg_face_spec;
+
+/* A convex cell: the constructor arguments (o, mat, div, gradT, T) of the
+ * reference's cell templates. */
+typedef struct mcg_subdomain_spec {
+    int32_t cell;              /* mcg_cell_kind                                 */
+    int32_t nface;             /* 6, 5 or 4 for the three cell kinds            */
+    double origin[3];
+    double mat[9];             /* column-major: columns are the cell edges      */
+    int64_t div[3];            /* tally divisions (subdomain.cpp:41-71)         */
+    double grad_t[3];          /* imposed gradient: volumetric emitter          */
+    mcg_face_spec face[6];     /* in the reference's container order            */
+} mcg_subdomain_spec;
+
+/* makePair for periodic faces (boundary.cpp:524-550: translation and
+ * rotation) or for interfaces (boundary.cpp:361-369; transl/rot unused). */
+typedef struct mcg_pair_spec {
+    int32_t sdom_a, face_a;
+    int32_t sdom_b, face_b;
+    double transl[3];
+    double rot[9];             /* column-major; all zeros means identity */
+} mcg_pair_spec;
+
+/* Extension for the porous-film family: a cylindrical pore whose axis is
+ * parallel to z, cut out of one subdomain.  The pore wall is a boundary of
+ * kind MCG_DIFF, MCG_SPEC or MCG_PARTIAL. */
+typedef struct mcg_pore_spec {
+    int32_t sdom;
+    int32_t kind;
+    double center[2];          /* axis position (x, y) */
+    double radius;
+    double specularity;
+} mcg_pore_spec;
+
+typedef struct mcg_domain_spec {
+    int32_t nsdom;
+    int32_t npair;
+    int32_t npore;
+    int32_t reserved;
+    const mcg_subdomain_spec* sdom;
+    const mcg_pair_spec* pair;
+    const mcg_pore_spec* pore;
+} mcg_domain_spec;
+
+/* -------------------------------------------------------------- problem */
+enum mcg_tally_kind {
+    MCG_TEMP = 0,      /* TempProblem     problem.cpp:468-481  rows 1          */
+    MCG_FLUX = 1,      /* FluxProblem     problem.cpp:501-514  rows 3          */
+    MCG_MULTI = 2,     /* MultiProblem    problem.cpp:534-551  rows 4          */
+    MCG_CUMTEMP = 3,   /* CumTempProblem  problem.cpp:557-599  rows size+1     */
+    MCG_CUMFLUX = 4    /* CumFluxProblem  problem.cpp:605-648  rows 3(size+1)  */
+};
+
+enum mcg_precision {
+    MCG_FP64 = 0,          /* flight, positions and tallies in FP64            */
+    MCG_FP32_FLIGHT = 1    /* positions, directions, flight times, velocities
+                              in FP32; tallies and sampling tables in FP64     */
+};
+
+/* FieldProblem(mat, dom, nemit, maxscat, maxloop) (problem.cpp:315-342)
+ * plus the engine's own run controls. */
+typedef struct mcg_problem_spec {
+    int32_t tally;         /* mcg_tally_kind                                   */
+    int32_t precision;     /* mcg_precision                                    */
+    int64_t nemit;         /* requested bundles (rounded per emitter)          */
+    int64_t maxscat;       /* scatter cap (intrinsic + diffuse events)         */
+    int64_t maxloop;       /* step cap; 0 means 100 * maxscat                  */
+    int64_t size;          /* CumTemp / CumFlux only                           */
+    uint64_t seed;         /* Philox key of the solve                          */
+    int64_t first_bundle;  /* bundles [first, first + count) are run; the      */
+    int64_t bundle_count;  /* normalisation always uses the full nemit.
+                              count = 0 means all of them                       */
+} mcg_problem_spec;
+
+/* What the constructor of FieldProblem derives, plus the output shape. */
+typedef struct mcg_problem_info {
+    int64_t nemit;         /* sum of the rounded per-emitter counts            */
+    int64_t rows;          /* tally rows                                       */
+    int64_t cols;          /* tally columns (cells over all subdomains)        */
+    int64_t maxloop;       /* effective step cap                               */
+    int64_t nemitter;      /* emitters in registry order (domain.cpp:88-102)   */
+    double power;          /* power_ (problem.cpp:341)                         */
+    double weight_sum;     /* sum of emitter weights                           */
+} mcg_problem_info;
+
+/* Optional per-bundle outputs (deterministic-mode parity data). Each array
+ * has bundle_count entries, indexed by bundle id - first_bundle. */
+typedef struct mcg_counts {
+    uint32_t* nscat;       /* Phonon::nscat at termination                     */
+    uint32_t* nbdry;       /* boundary events (advect returned a boundary)     */
+    uint32_t* nstep;       /* executed loop iterations (advect calls)          */
+    uint32_t* fate;        /* 0 absorbed, 1 escaped, 2 maxscat, 3 maxloop      */
+} mcg_counts;
+
+enum mcg_fate { MCG_ABSORBED = 0, MCG_ESCAPED = 1, MCG_MAXSCAT = 2, MCG_MAXLOOP = 3 };
+
+typedef struct mcg_stats {
+    int64_t bundles;       /* bundles run                                      */
+    int64_t steps;         /* executed advect calls (the scatter-step count)   */
+    int64_t escapes;       /* Progress::incrEsc count (problem.cpp:408-426)    */
+    int64_t absorbed;
+    int64_t truncated;     /* stopped by maxscat or maxloop                    */
+    double kernel_ms;      /* device time of the trajectory kernels            */
+    double wall_ms;        /* host wall time of the whole call                 */
+} mcg_stats;
+
+/* --------------------------------------------------------------- library */
+int mcg_abi_version(void);
+const char* mcg_last_error(void);
+
+/* Host-only (no GPU needed). */
+int mcg_material_compute(const mcg_material_spec* mat, mcg_material_tables* out);
+int mcg_problem_plan(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                     const mcg_problem_spec* prob, mcg_problem_info* info,
+                     double* emit_weight, int64_t* emit_count, int64_t nemitter_cap);
+int mcg_alias_build(const double* pdf, int64_t n, uint32_t* thresh, uint32_t* alias);
+void mcg_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Scene: material tables and geometry resident on the current GPU. */
+typedef struct mcg_scene mcg_scene;
+int mcg_scene_create(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                     mcg_scene** out);
+int mcg_scene_destroy(mcg_scene* scene);
+
+/* FieldProblem::solve (problem.cpp:370-445) for the whole bundle set:
+ * out receives rows x cols doubles, column-major, equal to the reference's
+ * return value power_ * postProc(data) / cellVol.  counts may be NULL. */
+int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob,
+              double* out, int64_t out_len, mcg_counts* counts, mcg_stats* stats);
+
+/* Trajectory of one bundle of a field solve, step by step: the GPU
+ * counterpart of TrajProblem::solve (problem.cpp:226-299), which prints the
+ * hops of one bundle.  Record i (MCG_TRACE_WIDTH doubles) describes loop
+ * iteration i: step, hit (plane index, -1 intrinsic, -2 escape, -3 pore),
+ * distance moved, position after the flight (x, y, z), direction, free path
+ * left, mode and scatter count after the event, subdomain, fate (-1 while
+ * alive), position after the event (x, y).  nrec receives the step count
+ * (capped at cap). */
+#define MCG_TRACE_WIDTH 16
+int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, double* rec,
+              int64_t cap, int64_t* nrec);
+
+/* --------------------------------------------------------------- batching
+ * Many independent solves in one persistent launch.  Each solve has its
+ * own scene, problem and output; results equal those of mcg_solve. */
+typedef struct mcg_tune {
+    int32_t block;             /* threads per block: 64, 128, 256, 512 (0 = auto) */
+    int32_t per_thread;        /* phonons carried per thread: 1, 2, 4, 8        */
+    int32_t chunk;             /* bundles claimed per wavefront grab            */
+    int32_t blocks_per_cu;     /* occupancy target (0 = auto)                   */
+} mcg_tune;
+
+int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
+                    double* const* outs, const int64_t* out_lens,
+                    mcg_stats* stats, const mcg_tune* tune);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCG_H */

@@ -1,0 +1,1045 @@
+// mcg_kernels.hip -- the trajectory kernel and the scene / solve entry points.
+//
+// One launch runs a whole field solve (FieldProblem::solve, problem.cpp:370-445)
+// for a bundle range.  The kernel is persistent and history-based: every lane
+// owns one live bundle in registers and, as soon as that bundle terminates,
+// takes the next bundle id from its wavefront's pool (refilled 256 ids at a
+// time with one global atomic), so lanes stay busy until the bundle set is
+// exhausted.  Each bundle's random numbers depend only on (seed, bundle id,
+// event index), which makes results independent of scheduling.
+//
+// Per workgroup, the scatter sampling table and the per-mode velocity and mean
+// free path live in LDS, as do the subdomain planes and, when small enough,
+// the tally: FP64 LDS atomics (ds_add_f64), with the interior of 1-D flights
+// added as a range through a difference array (two atomics instead of one per
+// crossed cell), flushed once per workgroup with global FP64 atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mcg_device.h"
+#include "mcg_scene.h"
+
+namespace mcg {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kChunk = 256;            // bundle ids claimed per wavefront grab
+constexpr int kMaxRows = 4;            // nonzero amount entries per step
+constexpr size_t kLdsTallyCap = 48 * 1024;
+
+enum Counter : int { C_NEXT = 0, C_STEPS, C_ESC, C_ABS, C_TRUNC, C_BUNDLES, C_COUNT };
+
+template <typename F>
+struct KParams {
+    const DevPlane<F>* planes;
+    const DevSdom<F>* sdoms;
+    const DevEmitter<F>* emitters;
+    const DevPore<F>* pores;
+    const int64_t* emit_cdf;
+    const int32_t* inter_list;
+    const F* mode_vel;
+    const F* mode_mfp;
+    const uint64_t* scat_alias;
+    const uint64_t* flux_alias;
+    int32_t nplanes, nsdom, nemitter, npore, nmodes;
+    int32_t tally, rows, cols;
+    int64_t step_cum, maxscat;
+    uint32_t maxloop;
+    uint32_t k0, k1;
+    int64_t first, count;
+    double* direct;   // rows x cols, row-major
+    double* diff;     // rows x (cols + 1), row-major
+    unsigned long long* counters;
+    uint32_t* c_nscat;
+    uint32_t* c_nbdry;
+    uint32_t* c_nstep;
+    uint32_t* c_fate;
+    int32_t lds_tally;
+    int64_t trace_id;      // bundle whose steps are recorded, -1 for none
+    double* trace;         // MCG_TRACE_WIDTH doubles per step
+    int64_t trace_cap;     // records available
+};
+
+// Tally sink: LDS when staged, global otherwise (both FP64 atomic adds).
+struct Sink {
+    double* direct;
+    double* diff;
+    int32_t cols;
+    __device__ __forceinline__ void add(int row, int col, double v) const {
+        unsafeAtomicAdd(direct + row * cols + col, v);
+    }
+    __device__ __forceinline__ void add_range(int row, int lo, int hi, double v) const {
+        // +v on columns lo..hi through the difference array
+        unsafeAtomicAdd(diff + row * (cols + 1) + lo, v);
+        unsafeAtomicAdd(diff + row * (cols + 1) + hi + 1, -v);
+    }
+};
+
+template <typename F>
+struct Lane {
+    dev::V3<F> pos, dir;
+    F scat_next;
+    uint32_t mode;
+    int32_t sdom;
+    int32_t sign;
+    uint32_t nscat, nbdry, nstep;
+    int64_t id;
+};
+
+template <typename F>
+__device__ __forceinline__ bool inside(const DevPlane<F>* planes, const DevSdom<F>& S,
+                                       const DevPore<F>* pores, const dev::V3<F>& p) {
+    for (int b = S.pbeg; b < S.pend; ++b) {
+        const DevPlane<F>& P = planes[b];
+        if (dev::dot3(P.n, p) + P.c < -S.eps) return false;
+    }
+    if (S.pore >= 0) {
+        const DevPore<F>& Q = pores[S.pore];
+        const F dx = p.x - Q.cx, dy = p.y - Q.cy;
+        const F rr = Q.r - Q.eps;
+        if (dx * dx + dy * dy < rr * rr) return false;
+    }
+    return true;
+}
+
+// Free path: l = v tau (-ln(1 - u)), redrawn while l < the smallest normal
+// (material.cpp:215-224).
+template <typename F>
+__device__ __forceinline__ F draw_path(const dev::Event& ev, const uint4& b1, F mfp) {
+    F d = mfp * dev::Real<F>::neglog1m(dev::Real<F>::uniform(b1.x, b1.y));
+    for (uint32_t k = 0; d < dev::Real<F>::kMin; ++k) {
+        const uint4 b = ev.block(4u + k / 2u);
+        const uint32_t a = (k & 1u) ? b.z : b.x, c = (k & 1u) ? b.w : b.y;
+        d = mfp * dev::Real<F>::neglog1m(dev::Real<F>::uniform(a, c));
+    }
+    return d;
+}
+
+// Emitter::emit with the wall / volume draws, then drawScatNext
+// (problem.cpp:386-399, boundary.cpp:378-431, subdomain.cpp:255-377).
+template <typename F>
+__device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* mfp_tab,
+                     int64_t n, Lane<F>& L) {
+    int lo = 0, hi = K.nemitter;   // upper_bound of n in the cumulative counts
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (K.emit_cdf[mid] <= n) lo = mid + 1; else hi = mid;
+    }
+    const DevEmitter<F>& E = K.emitters[lo];
+    const dev::Event ev{K.k0, K.k1, 0u, uint32_t(n), uint32_t(uint64_t(n) >> 32)};
+    const uint4 b0 = ev.block(0), b1 = ev.block(1), b2 = ev.block(2);
+    L.mode = dev::alias_draw(flux_alias, uint32_t(K.nmodes), b1.z);
+    dev::V3<F> d;
+    if (!E.volume) {
+        const F r1 = dev::Real<F>::uniform(b2.x, b2.y), r2 = dev::Real<F>::uniform(b2.z, b2.w);
+        F a = r1, b = r2;
+        if (E.shape == SHAPE_TRI && !(r1 + r2 < F(1))) {
+            a = F(1) - r1;
+            b = F(1) - r2;
+        }
+        L.pos = dev::V3<F>{E.o[0] + (a * E.si[0] + b * E.sj[0]), E.o[1] + (a * E.si[1] + b * E.sj[1]),
+                           E.o[2] + (a * E.si[2] + b * E.sj[2])};
+        d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, false));
+        L.sign = E.sign ? 1 : -1;
+    } else {
+        const uint4 b3 = ev.block(3);
+        F c0 = dev::Real<F>::uniform(b2.x, b2.y), c1 = dev::Real<F>::uniform(b2.z, b2.w),
+          c2 = dev::Real<F>::uniform(b3.x, b3.y);
+        if (E.shape != MCG_PARALLELEPIPED && c0 + c1 > F(1)) {
+            c0 = F(1) - c0;
+            c1 = F(1) - c1;
+        }
+        if (E.shape == MCG_TETRAHEDRON) {
+            if (c1 + c2 > F(1)) {
+                const F t = c2;
+                c2 = F(1) - c0 - c1;
+                c1 = F(1) - t;
+            } else if (c0 + c1 + c2 > F(1)) {
+                const F t = c2;
+                c2 = c0 + c1 + c2 - F(1);
+                c0 = F(1) - c1 - t;
+            }
+        }
+        const dev::V3<F> m = dev::matvec(E.mat, dev::V3<F>{c0, c1, c2});
+        L.pos = dev::V3<F>{E.o[0] + m.x, E.o[1] + m.y, E.o[2] + m.z};
+        d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, true));
+        L.sign = (d.x * E.grad[0] + d.y * E.grad[1] + d.z * E.grad[2] < F(0)) ? 1 : -1;
+    }
+    L.dir = dev::normalized(d);
+    L.sdom = E.sdom;
+    L.scat_next = draw_path<F>(ev, b1, mfp_tab[L.mode]);
+    L.nscat = L.nbdry = L.nstep = 0;
+    L.id = n;
+}
+
+// Field::accumulate (field.cpp:92-220) of `amt` (nv entries on rows r0..)
+// over the segment a -> b of subdomain S.
+template <typename F>
+__device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink,
+                                           const dev::V3<F>& a, const dev::V3<F>& b, int r0,
+                                           int nv, const double (&amt)[kMaxRows]) {
+    const int flag = S.accum;
+    if (flag < -1) return;
+    if (flag < 0) {
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j)
+            if (j < nv) sink.add(r0 + j, S.col_off, amt[j]);
+        return;
+    }
+    const dev::V3<F> la{a.x - S.o[0], a.y - S.o[1], a.z - S.o[2]};
+    const dev::V3<F> lb{b.x - S.o[0], b.y - S.o[1], b.z - S.o[2]};
+    const dev::V3<F> ia = dev::matvec(S.inv, la), ib = dev::matvec(S.inv, lb);
+    const F bc[3] = {S.divf[0] * ia.x, S.divf[1] * ia.y, S.divf[2] * ia.z};
+    const F ec[3] = {S.divf[0] * ib.x, S.divf[1] * ib.y, S.divf[2] * ib.z};
+    int bi[3], ei[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int vb = int(floor(bc[k])), ve = int(floor(ec[k]));
+        bi[k] = min(max(vb, 0), S.max[k]);
+        ei[k] = min(max(ve, 0), S.max[k]);
+    }
+    if (flag < 3) {
+        // one axis: select it without indexing the arrays at run time
+        const F bcd_f = flag == 0 ? bc[0] : flag == 1 ? bc[1] : bc[2];
+        const F ecd_f = flag == 0 ? ec[0] : flag == 1 ? ec[1] : ec[2];
+        const int bb = flag == 0 ? bi[0] : flag == 1 ? bi[1] : bi[2];
+        const int ee = flag == 0 ? ei[0] : flag == 1 ? ei[1] : ei[2];
+        if (bb == ee) {
+#pragma unroll
+            for (int j = 0; j < kMaxRows; ++j)
+                if (j < nv) sink.add(r0 + j, S.col_off + bb, amt[j]);
+            return;
+        }
+        const double bcd = double(bcd_f), ecd = double(ecd_f);
+        const double ad = fabs(double(ecd_f - bcd_f));
+        double fb, fe;
+        int lo, hi;
+        if (bb < ee) {
+            fb = double(1 + bb) - bcd;
+            fe = ecd - double(ee);
+            lo = bb + 1;
+            hi = ee - 1;
+        } else {
+            fb = bcd - double(bb);
+            fe = double(1 + ee) - ecd;
+            lo = ee + 1;
+            hi = bb - 1;
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j) {
+            if (j >= nv) continue;
+            const double ca = amt[j] / ad;
+            sink.add(r0 + j, S.col_off + bb, ca * fb);
+            sink.add(r0 + j, S.col_off + ee, ca * fe);
+            if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, ca);
+        }
+        return;
+    }
+    // 2-D / 3-D: walk the crossings in parameter order; crossings of several
+    // axes at exactly the same parameter are one step (the ordered map of
+    // field.cpp:156-218 merges equal keys).
+    F dc[3];
+    int pm[3], nxt[3], last[3];
+    bool live[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dc[k] = ec[k] - bc[k];
+        live[k] = false;
+        pm[k] = 0;
+        nxt[k] = last[k] = 0;
+        if (fabs(double(dc[k])) < DBL_MIN || bi[k] == ei[k]) continue;
+        live[k] = true;
+        if (bi[k] < ei[k]) {
+            pm[k] = 1;
+            nxt[k] = bi[k] + 1;
+            last[k] = ei[k];
+        } else {
+            pm[k] = -1;
+            nxt[k] = bi[k];
+            last[k] = ei[k] + 1;
+        }
+    }
+    int idx[3] = {bi[0], bi[1], bi[2]};
+    F prev = F(0);
+    for (;;) {
+        F pmin = F(2);
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (!live[k]) continue;
+            const F p = (F(nxt[k]) - bc[k]) / dc[k];
+            if (!any || p < pmin) pmin = p;
+            any = true;
+        }
+        if (!any) break;
+        const int col = S.col_off + idx[0] * S.stride[0] + idx[1] * S.stride[1] + idx[2] * S.stride[2];
+        const double seg = double(pmin - prev);
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j)
+            if (j < nv) sink.add(r0 + j, col, amt[j] * seg);
+        prev = pmin;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (!live[k]) continue;
+            const F p = (F(nxt[k]) - bc[k]) / dc[k];
+            if (p == pmin) {
+                idx[k] += pm[k];
+                if (nxt[k] == last[k]) live[k] = false;
+                nxt[k] += pm[k];
+            }
+        }
+    }
+    if (prev != F(1)) {
+        const int col = S.col_off + idx[0] * S.stride[0] + idx[1] * S.stride[1] + idx[2] * S.stride[2];
+        const double seg = double(F(1) - prev);
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j)
+            if (j < nv) sink.add(r0 + j, col, amt[j] * seg);
+    }
+}
+
+// Scatter off a cylindrical pore wall.  The wall normal at the hit point
+// points away from the axis (into the material); the reflection and the
+// Lambertian rotation are built from it as for a planar face.
+template <typename F>
+__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Event& ev, Lane<F>& L) {
+    const dev::V3<F> n = dev::normalized(dev::V3<F>{(L.pos.x - Q.cx) / Q.r, (L.pos.y - Q.cy) / Q.r, F(0)});
+    int kind = Q.kind;
+    if (kind == MCG_PARTIAL) kind = ev.block(1).w < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
+    if (kind == MCG_SPEC) {
+        const F nv[3] = {n.x, n.y, n.z};
+        F r[9];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) r[3 * c + q] = (q == c ? F(1) : F(0)) - F(2) * (nv[q] * nv[c]);
+        L.dir = dev::normalized(dev::matvec(r, L.dir));
+        return;
+    }
+    // rotation taking z onto n (n.z = 0, so never the antiparallel case)
+    const F s = dev::Real<F>::sqrt_((F(1) + n.z) * F(2));
+    const F invs = F(1) / s;
+    const F qx = -n.y * invs, qy = n.x * invs, qz = F(0) * invs, qw = s * F(0.5);
+    const F tx = F(2) * qx, ty = F(2) * qy, tz = F(2) * qz;
+    const F twx = tx * qw, twy = ty * qw, twz = tz * qw;
+    const F txx = tx * qx, txy = ty * qx, txz = tz * qx;
+    const F tyy = ty * qy, tyz = tz * qy, tzz = tz * qz;
+    const F r[9] = {F(1) - (tyy + tzz), txy + twz, txz - twy,
+                    txy - twz, F(1) - (txx + tzz), tyz + twx,
+                    txz + twy, tyz - twx, F(1) - (txx + tyy)};
+    L.dir = dev::normalized(dev::matvec(r, dev::draw_aniso<F>(ev.block(0), false)));
+    L.nscat++;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <typename F, bool kLdsTables, bool kLdsTally>
+__global__ void __launch_bounds__(256) trajectory_kernel(KParams<F> K) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    // ---- stage tables in LDS
+    const int ntab = kLdsTables ? K.nmodes : 0;
+    const uint64_t* s_scat = kLdsTables ? reinterpret_cast<const uint64_t*>(lds) : K.scat_alias;
+    const F* s_vel = kLdsTables ? reinterpret_cast<const F*>(lds + 8 * size_t(ntab)) : K.mode_vel;
+    const F* s_mfp = kLdsTables ? s_vel + ntab : K.mode_mfp;
+    size_t off = (size_t(ntab) * (8 + 2 * sizeof(F)) + 15) & ~size_t(15);
+    DevPlane<F>* s_planes = reinterpret_cast<DevPlane<F>*>(lds + off);
+    off = (off + sizeof(DevPlane<F>) * K.nplanes + 15) & ~size_t(15);
+    DevSdom<F>* s_sdoms = reinterpret_cast<DevSdom<F>*>(lds + off);
+    off = (off + sizeof(DevSdom<F>) * K.nsdom + 15) & ~size_t(15);
+    double* s_direct = reinterpret_cast<double*>(lds + off);
+    double* s_diff = s_direct + size_t(K.rows) * K.cols;
+    const int tid = threadIdx.x;
+    if (kLdsTables) {
+        uint64_t* w_scat = reinterpret_cast<uint64_t*>(lds);
+        F* w_vel = reinterpret_cast<F*>(lds + 8 * size_t(ntab));
+        F* w_mfp = w_vel + ntab;
+        for (int i = tid; i < ntab; i += blockDim.x) {
+            w_scat[i] = K.scat_alias[i];
+            w_vel[i] = K.mode_vel[i];
+            w_mfp[i] = K.mode_mfp[i];
+        }
+    }
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(K.planes);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_planes);
+        const int nw = int(sizeof(DevPlane<F>) / 4) * K.nplanes;
+        for (int i = tid; i < nw; i += blockDim.x) dst[i] = src[i];
+        src = reinterpret_cast<const uint32_t*>(K.sdoms);
+        dst = reinterpret_cast<uint32_t*>(s_sdoms);
+        const int ns = int(sizeof(DevSdom<F>) / 4) * K.nsdom;
+        for (int i = tid; i < ns; i += blockDim.x) dst[i] = src[i];
+    }
+    if (kLdsTally) {
+        const int n = K.rows * K.cols + K.rows * (K.cols + 1);
+        for (int i = tid; i < n; i += blockDim.x) s_direct[i] = 0.0;
+    }
+    const Sink sink = kLdsTally ? Sink{s_direct, s_diff, K.cols} : Sink{K.direct, K.diff, K.cols};
+    __syncthreads();
+
+    const int lane = tid & (kWave - 1);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    int64_t pool_base = 0, pool_lim = 0;   // wave-uniform id pool
+    bool active = false, finished = false;
+    Lane<F> L;
+    unsigned long long n_steps = 0, n_esc = 0, n_abs = 0, n_trunc = 0, n_bundles = 0;
+
+    for (;;) {
+        // ---- refill lanes whose bundle ended
+        const bool need = !active && !finished;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const int cnt = __popcll(mask);
+            const int rank = __popcll(mask & lt_mask);
+            const int64_t avail = pool_lim - pool_base;
+            int64_t mine;
+            if (avail >= cnt) {
+                mine = pool_base + rank;
+                pool_base += cnt;
+            } else {
+                const int leader = __ffsll((long long)mask) - 1;
+                unsigned long long fresh = 0;
+                if (lane == leader) fresh = atomicAdd(K.counters + C_NEXT, (unsigned long long)kChunk);
+                fresh = __shfl(fresh, leader);
+                mine = rank < avail ? pool_base + rank : int64_t(fresh) + (rank - avail);
+                pool_base = int64_t(fresh) + (cnt - avail);
+                pool_lim = int64_t(fresh) + kChunk;
+            }
+            if (need) {
+                if (mine < K.count) {
+                    emit<F>(K, K.flux_alias, s_mfp, K.first + mine, L);
+                    active = true;
+                } else {
+                    finished = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (!active) continue;
+
+        // ---- one scatter-step (problem.cpp:401-435)
+        const DevSdom<F>& S = s_sdoms[L.sdom];
+        const F vel = s_vel[L.mode];
+        // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
+        F dmin = L.scat_next;
+        int hit = -1;
+        for (int b = S.pbeg; b < S.pend; ++b) {
+            const DevPlane<F>& P = s_planes[b];
+            const F nd = dev::dot3(P.n, L.dir);
+            if (nd >= F(0)) continue;
+            const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
+            if (t < dmin) {
+                dmin = t;
+                hit = b;
+            }
+        }
+        if (S.pore >= 0) {
+            const DevPore<F>& Q = K.pores[S.pore];
+            const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
+            const F bq = dx * L.dir.x + dy * L.dir.y;
+            if (bq < F(0)) {
+                const F aq = L.dir.x * L.dir.x + L.dir.y * L.dir.y;
+                const F cq = dx * dx + dy * dy - Q.r2;
+                const F disc = bq * bq - aq * cq;
+                if (disc >= F(0)) {
+                    const F t = cq / (dev::Real<F>::sqrt_(disc) - bq);
+                    if (t < dmin) {
+                        dmin = t;
+                        hit = -3;
+                    }
+                }
+            }
+        }
+        L.scat_next -= dmin;
+        if (L.scat_next < dev::Real<F>::kMin) L.scat_next = F(0);
+        const dev::V3<F> pre = L.pos;
+        L.pos = dev::V3<F>{pre.x + dmin * L.dir.x, pre.y + dmin * L.dir.y, pre.z + dmin * L.dir.z};
+        L.nstep++;
+        const dev::V3<F> apos = L.pos;
+        uint32_t fate = 0xFFFFFFFFu;
+        if (dmin < -S.eps || !inside(s_planes, S, K.pores, L.pos)) {
+            fate = MCG_ESCAPED;
+        } else {
+            // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637)
+            const double sg = double(L.sign);
+            const double dt = sg * double(dmin / vel);
+            const double dx = sg * double(L.pos.x - pre.x);
+            const double dy = sg * double(L.pos.y - pre.y);
+            const double dz = sg * double(L.pos.z - pre.z);
+            const int tk = K.tally;
+            const bool with_t = tk == MCG_TEMP || tk == MCG_MULTI || tk == MCG_CUMTEMP;
+            const double amt[kMaxRows] = {with_t ? dt : dx, with_t ? dx : dy, with_t ? dy : dz, dz};
+            const int nv = tk == MCG_MULTI ? 4 : (with_t ? 1 : 3);
+            int r0 = 0;
+            if (tk >= MCG_CUMTEMP) {
+                const int bin = int((int64_t(L.nscat) + K.step_cum - 1) / K.step_cum);
+                r0 = tk == MCG_CUMTEMP ? bin : 3 * bin;
+            }
+            accumulate<F>(S, sink, pre, L.pos, r0, nv, amt);
+
+            // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
+            const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
+            if (hit == -1) {
+                const uint4 b0 = ev.block(0), b1 = ev.block(1);
+                L.mode = dev::alias_draw(s_scat, uint32_t(K.nmodes), b1.z);
+                L.dir = dev::normalized(dev::draw_iso<F>(b0));
+                L.nscat++;
+                L.scat_next = draw_path<F>(ev, b1, s_mfp[L.mode]);
+            } else if (hit >= 0) {
+                L.nbdry++;
+                const DevPlane<F>& P = s_planes[hit];
+                int kind = P.kind;
+                if (kind == MCG_PARTIAL) kind = ev.block(1).w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+                switch (kind) {
+                    case MCG_SPEC:   // boundary.cpp:283-287
+                        L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
+                        break;
+                    case MCG_DIFF:   // boundary.cpp:308-312
+                        L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(ev.block(0), false)));
+                        L.nscat++;
+                        break;
+                    case MCG_ISOT:   // boundary.cpp:456-460
+                        fate = MCG_ABSORBED;
+                        break;
+                    case MCG_PERI: { // boundary.cpp:517-522
+                        const dev::V3<F> q = dev::matvec(P.prot, L.pos);
+                        L.pos = dev::V3<F>{q.x + P.pt[0], q.y + P.pt[1], q.z + P.pt[2]};
+                        L.dir = dev::normalized(dev::matvec(P.prot, L.dir));
+                        L.sdom = P.target;
+                        break;
+                    }
+                    default: {       // MCG_INTER, boundary.cpp:349-359
+                        int target = -1;
+                        if (P.icnt == 1) {
+                            target = K.inter_list[P.ibeg];
+                        } else {
+                            for (int q = 0; q < P.icnt; ++q) {
+                                const int cand = K.inter_list[P.ibeg + q];
+                                if (inside(s_planes, s_sdoms[cand], K.pores, L.pos)) {
+                                    target = cand;
+                                    break;
+                                }
+                            }
+                        }
+                        if (target < 0) fate = MCG_ESCAPED;
+                        else L.sdom = target;
+                        break;
+                    }
+                }
+            } else {
+                L.nbdry++;
+                pore_scatter<F>(K.pores[S.pore], ev, L);
+            }
+            if (fate == 0xFFFFFFFFu) {
+                if (int64_t(L.nscat) >= K.maxscat) fate = MCG_MAXSCAT;
+                else if (L.nstep >= K.maxloop) fate = MCG_MAXLOOP;
+            }
+        }
+        if (L.id == K.trace_id && int64_t(L.nstep) <= K.trace_cap) {
+            double* r = K.trace + (int64_t(L.nstep) - 1) * MCG_TRACE_WIDTH;
+            r[0] = double(L.nstep);
+            r[1] = double(hit);
+            r[2] = double(dmin);
+            r[3] = double(apos.x);
+            r[4] = double(apos.y);
+            r[5] = double(apos.z);
+            r[6] = double(L.dir.x);
+            r[7] = double(L.dir.y);
+            r[8] = double(L.dir.z);
+            r[9] = double(L.scat_next);
+            r[10] = double(L.mode);
+            r[11] = double(L.nscat);
+            r[12] = double(L.sdom);
+            r[13] = fate == 0xFFFFFFFFu ? -1.0 : double(fate);
+            r[14] = double(L.pos.x);
+            r[15] = double(L.pos.y);
+        }
+        if (fate != 0xFFFFFFFFu) {
+            n_steps += L.nstep;
+            n_bundles++;
+            n_esc += fate == MCG_ESCAPED;
+            n_abs += fate == MCG_ABSORBED;
+            n_trunc += fate == MCG_MAXSCAT || fate == MCG_MAXLOOP;
+            if (K.c_nscat) {
+                const int64_t k = L.id - K.first;
+                K.c_nscat[k] = L.nscat;
+                K.c_nbdry[k] = L.nbdry;
+                K.c_nstep[k] = L.nstep;
+                K.c_fate[k] = fate;
+            }
+            active = false;
+        }
+    }
+
+    // ---- counters and tally flush
+    n_steps = wave_sum(n_steps);
+    n_esc = wave_sum(n_esc);
+    n_abs = wave_sum(n_abs);
+    n_trunc = wave_sum(n_trunc);
+    n_bundles = wave_sum(n_bundles);
+    if (lane == 0) {
+        atomicAdd(K.counters + C_STEPS, n_steps);
+        atomicAdd(K.counters + C_ESC, n_esc);
+        atomicAdd(K.counters + C_ABS, n_abs);
+        atomicAdd(K.counters + C_TRUNC, n_trunc);
+        atomicAdd(K.counters + C_BUNDLES, n_bundles);
+    }
+    if (kLdsTally) {
+        __syncthreads();
+        const int nd = K.rows * K.cols;
+        for (int i = tid; i < nd; i += blockDim.x) {
+            const double v = s_direct[i];
+            if (v != 0.0) unsafeAtomicAdd(K.direct + i, v);
+        }
+        const int nf = K.rows * (K.cols + 1);
+        for (int i = tid; i < nf; i += blockDim.x) {
+            const double v = s_diff[i];
+            if (v != 0.0) unsafeAtomicAdd(K.diff + i, v);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host
+thread_local std::string t_err;
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::mcg::set_error(MCG_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (count == 0) return MCG_OK;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) { p = nullptr; return set_error(MCG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+        return MCG_OK;
+    }
+    int upload(const T* src, size_t count) {
+        int rc = alloc(count);
+        if (rc || count == 0) return rc;
+        hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return set_error(MCG_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return MCG_OK;
+    }
+};
+
+template <typename F>
+struct DevScene {
+    DevBuf<DevPlane<F>> planes;
+    DevBuf<DevSdom<F>> sdoms;
+    DevBuf<DevEmitter<F>> emitters;
+    DevBuf<DevPore<F>> pores;
+    DevBuf<F> mode_vel, mode_mfp;
+};
+
+template <typename F, typename S>
+void convert(const S& src, F& dst);
+
+template <typename F>
+void convert_plane(const DevPlane<double>& s, DevPlane<F>& d) {
+    for (int i = 0; i < 3; ++i) { d.n[i] = F(s.n[i]); d.pt[i] = F(s.pt[i]); }
+    d.c = F(s.c);
+    for (int i = 0; i < 9; ++i) { d.rot[i] = F(s.rot[i]); d.refl[i] = F(s.refl[i]); d.prot[i] = F(s.prot[i]); }
+    d.kind = s.kind; d.sdom = s.sdom; d.target = s.target; d.ibeg = s.ibeg; d.icnt = s.icnt; d.spec_thr = s.spec_thr;
+}
+
+template <typename F>
+void convert_sdom(const DevSdom<double>& s, DevSdom<F>& d, double eps_scale) {
+    for (int i = 0; i < 3; ++i) { d.o[i] = F(s.o[i]); d.divf[i] = F(s.divf[i]); d.max[i] = s.max[i]; d.stride[i] = s.stride[i]; }
+    for (int i = 0; i < 9; ++i) d.inv[i] = F(s.inv[i]);
+    d.eps = F(s.eps * eps_scale);
+    d.accum = s.accum; d.pbeg = s.pbeg; d.pend = s.pend; d.col_off = s.col_off; d.pore = s.pore; d.pad = 0;
+}
+
+template <typename F>
+void convert_emitter(const DevEmitter<double>& s, DevEmitter<F>& d) {
+    for (int i = 0; i < 3; ++i) { d.o[i] = F(s.o[i]); d.si[i] = F(s.si[i]); d.sj[i] = F(s.sj[i]); d.grad[i] = F(s.grad[i]); }
+    for (int i = 0; i < 9; ++i) { d.mat[i] = F(s.mat[i]); d.rot[i] = F(s.rot[i]); }
+    d.volume = s.volume; d.shape = s.shape; d.sdom = s.sdom; d.sign = s.sign;
+}
+
+template <typename F>
+void convert_pore(const DevPore<double>& s, DevPore<F>& d, double eps_scale) {
+    d.cx = F(s.cx); d.cy = F(s.cy); d.r = F(s.r); d.r2 = F(s.r2); d.eps = F(s.eps * eps_scale);
+    d.kind = s.kind; d.spec_thr = s.spec_thr;
+}
+
+template <typename F>
+int upload_scene(const HostScene& h, DevScene<F>& d) {
+    // The FP32 flight path scales the kill tolerance with FLT_EPSILON
+    // instead of DBL_EPSILON (100 eps |M|, subdomain.cpp:45).
+    const double eps_scale = sizeof(F) == 4 ? double(FLT_EPSILON) / DBL_EPSILON : 1.0;
+    std::vector<DevPlane<F>> pl(h.planes.size());
+    for (size_t i = 0; i < pl.size(); ++i) convert_plane(h.planes[i], pl[i]);
+    std::vector<DevSdom<F>> sd(h.sdoms.size());
+    for (size_t i = 0; i < sd.size(); ++i) convert_sdom(h.sdoms[i], sd[i], eps_scale);
+    std::vector<DevEmitter<F>> em(h.emitters.size());
+    for (size_t i = 0; i < em.size(); ++i) convert_emitter(h.emitters[i], em[i]);
+    std::vector<DevPore<F>> po(h.pores.size());
+    for (size_t i = 0; i < po.size(); ++i) convert_pore(h.pores[i], po[i], eps_scale);
+    std::vector<F> mv(h.mode_vel.size()), mm(h.mode_mfp.size());
+    for (size_t i = 0; i < mv.size(); ++i) { mv[i] = F(h.mode_vel[i]); mm[i] = F(h.mode_mfp[i]); }
+    int rc;
+    if ((rc = d.planes.upload(pl.data(), pl.size()))) return rc;
+    if ((rc = d.sdoms.upload(sd.data(), sd.size()))) return rc;
+    if ((rc = d.emitters.upload(em.data(), em.size()))) return rc;
+    if ((rc = d.pores.upload(po.data(), po.size()))) return rc;
+    if ((rc = d.mode_vel.upload(mv.data(), mv.size()))) return rc;
+    if ((rc = d.mode_mfp.upload(mm.data(), mm.size()))) return rc;
+    return MCG_OK;
+}
+
+}  // namespace
+
+int set_error(int code, const std::string& msg) {
+    t_err = msg;
+    return code;
+}
+
+}  // namespace mcg
+
+struct mcg_scene {
+    mcg::HostScene host;
+    mcg::DevScene<double> d64;
+    mcg::DevScene<float> d32;
+    mcg::DevBuf<uint64_t> scat_alias, flux_alias;
+    mcg::DevBuf<int32_t> inter_list;
+    int device = 0;
+    int num_cu = 0;
+    size_t lds_limit = 0;
+};
+
+namespace mcg {
+namespace {
+
+template <typename F>
+size_t lds_bytes(const HostScene& h, int rows, bool tables, bool tally) {
+    size_t off = ((tables ? h.mode_cell.size() : 0) * (8 + 2 * sizeof(F)) + 15) & ~size_t(15);
+    off = (off + sizeof(DevPlane<F>) * h.planes.size() + 15) & ~size_t(15);
+    off = (off + sizeof(DevSdom<F>) * h.sdoms.size() + 15) & ~size_t(15);
+    if (tally) off += sizeof(double) * (size_t(rows) * h.cols + size_t(rows) * (h.cols + 1));
+    return off;
+}
+
+template <typename F>
+int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* counts, mcg_stats* stats,
+              int64_t trace_id = -1, double* trace = nullptr, int64_t trace_cap = 0) {
+    const HostScene& h = sc->host;
+    DevScene<F>& d = sizeof(F) == 8 ? reinterpret_cast<DevScene<F>&>(sc->d64)
+                                    : reinterpret_cast<DevScene<F>&>(sc->d32);
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(sc->device));
+    const int rows = int(p.rows);
+    const int cols = int(h.cols);
+    if (p.rows > (1 << 20) || h.cols * p.rows > (1ll << 31)) return set_error(MCG_ERANGE, "tally too large");
+    if (p.maxloop <= 0) return set_error(MCG_EINVAL, "maxloop must be positive");
+    DevBuf<double> direct, diff;
+    DevBuf<unsigned long long> ctr;
+    DevBuf<int64_t> cdf;
+    DevBuf<uint32_t> cnt;
+    int rc;
+    if ((rc = direct.alloc(size_t(rows) * cols))) return rc;
+    if ((rc = diff.alloc(size_t(rows) * (cols + 1)))) return rc;
+    if ((rc = ctr.alloc(C_COUNT))) return rc;
+    if ((rc = cdf.upload(p.emit_cdf.data(), p.emit_cdf.size()))) return rc;
+    const bool want_counts = counts && counts->nscat && counts->nbdry && counts->nstep && counts->fate;
+    if (want_counts && (rc = cnt.alloc(size_t(4) * p.count))) return rc;
+    HIP_TRY(hipMemset(direct.p, 0, direct.n * sizeof(double)));
+    HIP_TRY(hipMemset(diff.p, 0, diff.n * sizeof(double)));
+    HIP_TRY(hipMemset(ctr.p, 0, ctr.n * sizeof(unsigned long long)));
+
+    KParams<F> K{};
+    K.planes = d.planes.p;
+    K.sdoms = d.sdoms.p;
+    K.emitters = d.emitters.p;
+    K.pores = d.pores.p;
+    K.emit_cdf = cdf.p;
+    K.inter_list = sc->inter_list.p;
+    K.mode_vel = d.mode_vel.p;
+    K.mode_mfp = d.mode_mfp.p;
+    K.scat_alias = sc->scat_alias.p;
+    K.flux_alias = sc->flux_alias.p;
+    K.nplanes = int32_t(h.planes.size());
+    K.nsdom = int32_t(h.sdoms.size());
+    K.nemitter = int32_t(h.emitters.size());
+    K.npore = int32_t(h.pores.size());
+    K.nmodes = int32_t(h.mode_cell.size());
+    K.tally = p.tally;
+    K.rows = rows;
+    K.cols = cols;
+    K.step_cum = p.step > 0 ? p.step : 1;
+    K.maxscat = p.maxscat;
+    K.maxloop = uint32_t(std::min<int64_t>(p.maxloop, 0xFFFFFFFEll));
+    K.k0 = uint32_t(p.seed);
+    K.k1 = uint32_t(p.seed >> 32);
+    K.first = p.first;
+    K.count = p.count;
+    K.direct = direct.p;
+    K.diff = diff.p;
+    K.counters = ctr.p;
+    if (want_counts) {
+        K.c_nscat = cnt.p;
+        K.c_nbdry = cnt.p + p.count;
+        K.c_nstep = cnt.p + 2 * p.count;
+        K.c_fate = cnt.p + 3 * p.count;
+    }
+    DevBuf<double> tbuf;
+    K.trace_id = -1;
+    if (trace && trace_cap > 0) {
+        if ((rc = tbuf.alloc(size_t(trace_cap) * MCG_TRACE_WIDTH))) return rc;
+        HIP_TRY(hipMemset(tbuf.p, 0, tbuf.n * sizeof(double)));
+        K.trace_id = trace_id;
+        K.trace = tbuf.p;
+        K.trace_cap = trace_cap;
+    }
+    // LDS plan: tables first, then the tally if it is small enough
+    const bool tables = lds_bytes<F>(h, rows, true, false) <= sc->lds_limit;
+    const size_t base = lds_bytes<F>(h, rows, tables, false);
+    const size_t with_tally = lds_bytes<F>(h, rows, tables, true);
+    const bool tally = with_tally - base <= kLdsTallyCap && with_tally <= sc->lds_limit;
+    const size_t lds = tally ? with_tally : base;
+    if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
+    K.lds_tally = tally ? 1 : 0;
+    const void* fn = tables ? (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, true, true>)
+                                     : reinterpret_cast<const void*>(trajectory_kernel<F, true, false>))
+                            : (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, false, true>)
+                                     : reinterpret_cast<const void*>(trajectory_kernel<F, false, false>));
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    const int block = 256;
+    int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
+    blocks_per_cu = std::min(blocks_per_cu, 4);
+    const int64_t needed = (p.count + block - 1) / block;
+    const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, 0));
+    if (tables && tally)
+        hipLaunchKernelGGL((trajectory_kernel<F, true, true>), dim3(grid), dim3(block), lds, 0, K);
+    else if (tables)
+        hipLaunchKernelGGL((trajectory_kernel<F, true, false>), dim3(grid), dim3(block), lds, 0, K);
+    else if (tally)
+        hipLaunchKernelGGL((trajectory_kernel<F, false, true>), dim3(grid), dim3(block), lds, 0, K);
+    else
+        hipLaunchKernelGGL((trajectory_kernel<F, false, false>), dim3(grid), dim3(block), lds, 0, K);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    std::vector<double> hd(direct.n), hf(diff.n);
+    unsigned long long hc[C_COUNT];
+    HIP_TRY(hipMemcpy(hd.data(), direct.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hf.data(), diff.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
+    if (want_counts) {
+        HIP_TRY(hipMemcpy(counts->nscat, cnt.p, p.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(counts->nbdry, cnt.p + p.count, p.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(counts->nstep, cnt.p + 2 * p.count, p.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(counts->fate, cnt.p + 3 * p.count, p.count * 4, hipMemcpyDeviceToHost));
+    }
+    if (trace && trace_cap > 0)
+        HIP_TRY(hipMemcpy(trace, tbuf.p, tbuf.n * sizeof(double), hipMemcpyDeviceToHost));
+    finalise_field(h, p, hd.data(), hf.data(), out);
+    if (stats) {
+        stats->bundles = int64_t(hc[C_BUNDLES]);
+        stats->steps = int64_t(hc[C_STEPS]);
+        stats->escapes = int64_t(hc[C_ESC]);
+        stats->absorbed = int64_t(hc[C_ABS]);
+        stats->truncated = int64_t(hc[C_TRUNC]);
+        stats->kernel_ms = ms;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (int64_t(hc[C_BUNDLES]) != p.count)
+        return set_error(MCG_EDEVICE, "kernel finished with bundles unaccounted for");
+    return MCG_OK;
+}
+
+}  // namespace
+}  // namespace mcg
+
+// ====================================================================== C ABI
+extern "C" {
+
+int mcg_abi_version(void) { return MCG_ABI_VERSION; }
+
+const char* mcg_last_error(void) { return mcg::t_err.c_str(); }
+
+void mcg_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    // host evaluation of the device generator's rounds (for tests)
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0, n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = uint32_t(p1);
+        c2 = n2;
+        c3 = uint32_t(p0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+int mcg_alias_build(const double* pdf, int64_t n, uint32_t* thresh, uint32_t* alias) {
+    if (!pdf || !thresh || !alias) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    std::vector<uint64_t> t;
+    int rc = mcg::build_alias(pdf, n, t);
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; ++i) {
+        thresh[i] = uint32_t(t[i]);
+        alias[i] = uint32_t(t[i] >> 32);
+    }
+    return MCG_OK;
+}
+
+int mcg_material_compute(const mcg_material_spec* mat, mcg_material_tables* out) {
+    if (!out) return mcg::set_error(MCG_EINVAL, "NULL output");
+    mcg::HostScene s;
+    int rc = mcg::build_material(mat, s);
+    if (rc) return rc;
+    const size_t n = size_t(s.nw * s.np) * sizeof(double);
+    if (out->tau) std::memcpy(out->tau, s.tau.data(), n);
+    if (out->vel) std::memcpy(out->vel, s.vel.data(), n);
+    if (out->energy_pdf) std::memcpy(out->energy_pdf, s.energy.data(), n);
+    if (out->flux_pdf) std::memcpy(out->flux_pdf, s.flux.data(), n);
+    if (out->scat_pdf) std::memcpy(out->scat_pdf, s.scat.data(), n);
+    out->energy_sum = s.energy_sum;
+    out->flux_sum = s.flux_sum;
+    out->scat_sum = s.scat_sum;
+    out->cond = s.cond;
+    out->nmodes = int64_t(s.mode_cell.size());
+    return MCG_OK;
+}
+
+int mcg_problem_plan(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                     const mcg_problem_spec* prob, mcg_problem_info* info, double* emit_weight,
+                     int64_t* emit_count, int64_t nemitter_cap) {
+    if (!info) return mcg::set_error(MCG_EINVAL, "NULL output");
+    mcg::HostScene s;
+    mcg::HostProblem p;
+    int rc = mcg::build_material(mat, s);
+    if (!rc) rc = mcg::build_domain(dom, s);
+    if (!rc) rc = mcg::build_problem(prob, s, p);
+    if (rc) return rc;
+    info->nemit = p.nemit;
+    info->rows = p.rows;
+    info->cols = s.cols;
+    info->maxloop = p.maxloop;
+    info->nemitter = int64_t(s.emitters.size());
+    info->power = p.power;
+    info->weight_sum = p.weight_sum;
+    for (int64_t i = 0; i < info->nemitter && i < nemitter_cap; ++i) {
+        if (emit_weight) emit_weight[i] = s.emit_weight[i];
+        if (emit_count) emit_count[i] = p.emit_count[i];
+    }
+    return MCG_OK;
+}
+
+int mcg_scene_create(const mcg_material_spec* mat, const mcg_domain_spec* dom, mcg_scene** out) {
+    if (!out) return mcg::set_error(MCG_EINVAL, "NULL output");
+    *out = nullptr;
+    std::unique_ptr<mcg_scene> sc(new (std::nothrow) mcg_scene());
+    if (!sc) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+    int rc = mcg::build_material(mat, sc->host);
+    if (!rc) rc = mcg::build_domain(dom, sc->host);
+    if (rc) return rc;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return mcg::set_error(MCG_EDEVICE, "no HIP device available");
+    HIP_TRY(hipGetDevice(&sc->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, sc->device));
+    sc->num_cu = prop.multiProcessorCount;
+    sc->lds_limit = prop.sharedMemPerBlock;
+    if ((rc = mcg::upload_scene(sc->host, sc->d64))) return rc;
+    if ((rc = mcg::upload_scene(sc->host, sc->d32))) return rc;
+    if ((rc = sc->scat_alias.upload(sc->host.scat_alias.data(), sc->host.scat_alias.size()))) return rc;
+    if ((rc = sc->flux_alias.upload(sc->host.flux_alias.data(), sc->host.flux_alias.size()))) return rc;
+    if ((rc = sc->inter_list.upload(sc->host.inter_list.data(), sc->host.inter_list.size()))) return rc;
+    *out = sc.release();
+    return MCG_OK;
+}
+
+int mcg_scene_destroy(mcg_scene* scene) {
+    delete scene;
+    return MCG_OK;
+}
+
+int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob, double* out, int64_t out_len,
+              mcg_counts* counts, mcg_stats* stats) {
+    if (!scene || !prob || !out) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    mcg::HostProblem p;
+    int rc = mcg::build_problem(prob, scene->host, p);
+    if (rc) return rc;
+    if (out_len < p.rows * scene->host.cols) return mcg::set_error(MCG_ERANGE, "output buffer too small");
+    if (p.count == 0) {
+        std::vector<double> z(p.rows * scene->host.cols, 0.0), zf(p.rows * (scene->host.cols + 1), 0.0);
+        mcg::finalise_field(scene->host, p, z.data(), zf.data(), out);
+        if (stats) *stats = mcg_stats{};
+        return MCG_OK;
+    }
+    if (p.precision == MCG_FP64) return mcg::run_solve<double>(scene, p, out, counts, stats);
+    return mcg::run_solve<float>(scene, p, out, counts, stats);
+}
+
+int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, double* rec,
+              int64_t cap, int64_t* nrec) {
+    if (!scene || !prob || !rec || !nrec || cap <= 0) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    mcg_problem_spec one = *prob;
+    mcg::HostProblem p;
+    int rc = mcg::build_problem(&one, scene->host, p);
+    if (rc) return rc;
+    if (bundle < 0 || bundle >= p.nemit) return mcg::set_error(MCG_ERANGE, "bundle outside [0, nemit)");
+    one.first_bundle = bundle;
+    one.bundle_count = 1;
+    if ((rc = mcg::build_problem(&one, scene->host, p))) return rc;
+    std::vector<double> out(size_t(p.rows * scene->host.cols));
+    uint32_t c_nscat, c_nbdry, c_nstep, c_fate;
+    mcg_counts counts{&c_nscat, &c_nbdry, &c_nstep, &c_fate};
+    rc = p.precision == MCG_FP64
+             ? mcg::run_solve<double>(scene, p, out.data(), &counts, nullptr, bundle, rec, cap)
+             : mcg::run_solve<float>(scene, p, out.data(), &counts, nullptr, bundle, rec, cap);
+    if (rc) return rc;
+    *nrec = std::min<int64_t>(int64_t(c_nstep), cap);
+    return MCG_OK;
+}
+
+int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
+                    double* const* outs, const int64_t* out_lens, mcg_stats* stats,
+                    const mcg_tune* tune) {
+    (void)tune;
+    if (n < 0 || (n > 0 && (!scenes || !probs || !outs || !out_lens)))
+        return mcg::set_error(MCG_EINVAL, "NULL argument");
+    for (int32_t i = 0; i < n; ++i) {
+        int rc = mcg_solve(scenes[i], &probs[i], outs[i], out_lens[i], nullptr, stats ? &stats[i] : nullptr);
+        if (rc) return rc;
+    }
+    return MCG_OK;
+}
+
+}  // extern "C"

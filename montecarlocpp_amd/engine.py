@@ -1,0 +1,152 @@
+"""ctypes binding of the engine library (montecarlocpp_amd/libmcg.so).
+
+The library is built in-tree by :mod:`montecarlocpp_amd.build`; this module
+never builds it and never falls back to another implementation: if the
+library is missing, :func:`lib` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmcg.so")
+
+# every function include/mcg.h declares
+EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_problem_plan",
+           "mcg_alias_build", "mcg_philox4x32_10", "mcg_scene_create", "mcg_scene_destroy",
+           "mcg_solve", "mcg_trace", "mcg_batch_solve")
+
+TRACE_WIDTH = 16
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mcg status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(abi.MCG_EDEVICE,
+                              f"{LIB_PATH} is not built; run python -m montecarlocpp_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.mcg_last_error.restype = C.c_char_p
+        L.mcg_abi_version.restype = C.c_int
+        MS, DS, PS = (C.POINTER(abi.MaterialSpec), C.POINTER(abi.DomainSpec),
+                      C.POINTER(abi.ProblemSpec))
+        L.mcg_material_compute.argtypes = [MS, C.POINTER(abi.MaterialTables)]
+        L.mcg_problem_plan.argtypes = [MS, DS, PS, C.POINTER(abi.ProblemInfo),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int64]
+        L.mcg_alias_build.argtypes = [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32)]
+        L.mcg_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.mcg_philox4x32_10.restype = None
+        L.mcg_scene_create.argtypes = [MS, DS, C.POINTER(C.c_void_p)]
+        L.mcg_scene_destroy.argtypes = [C.c_void_p]
+        L.mcg_solve.argtypes = [C.c_void_p, PS, C.POINTER(C.c_double), C.c_int64,
+                                C.POINTER(abi.Counts), C.POINTER(abi.Stats)]
+        L.mcg_trace.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double), C.c_int64,
+                                C.POINTER(C.c_int64)]
+        L.mcg_batch_solve.argtypes = [C.c_int32, C.POINTER(C.c_void_p), PS,
+                                      C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
+                                      C.POINTER(abi.Stats), C.POINTER(abi.Tune)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise EngineError(rc, lib().mcg_last_error().decode())
+
+
+def plan(mat, dom, prob):
+    info = abi.ProblemInfo()
+    w = np.zeros(256)
+    n = np.zeros(256, np.int64)
+    check(lib().mcg_problem_plan(C.byref(mat.spec), C.byref(dom.spec()), C.byref(prob),
+                                 C.byref(info), abi.dptr(w),
+                                 n.ctypes.data_as(C.POINTER(C.c_int64)), 256))
+    k = info.nemitter
+    return info, w[:k].copy(), n[:k].copy()
+
+
+def material_tables(mat):
+    n = mat.nw * mat.np
+    names = ("tau", "vel", "energy_pdf", "flux_pdf", "scat_pdf")
+    arrs = {k: np.zeros(n) for k in names}
+    t = abi.MaterialTables(*(abi.dptr(arrs[k]) for k in names))
+    check(lib().mcg_material_compute(C.byref(mat.spec), C.byref(t)))
+    arrs.update(energy_sum=t.energy_sum, flux_sum=t.flux_sum, scat_sum=t.scat_sum, cond=t.cond,
+                nmodes=t.nmodes)
+    return arrs
+
+
+def alias(pdf):
+    pdf = np.ascontiguousarray(pdf, np.float64)
+    thr = np.zeros(pdf.size, np.uint32)
+    al = np.zeros(pdf.size, np.uint32)
+    check(lib().mcg_alias_build(abi.dptr(pdf), pdf.size, abi.u32ptr(thr), abi.u32ptr(al)))
+    return thr, al
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().mcg_philox4x32_10(c, k, o)
+    return list(o)
+
+
+class Scene:
+    """Material tables + geometry resident on the current GPU
+    (mcg_scene_create).  Keeps the material and domain alive."""
+
+    def __init__(self, mat, dom):
+        self.mat, self.dom = mat, dom
+        h = C.c_void_p()
+        check(lib().mcg_scene_create(C.byref(mat.spec), C.byref(dom.spec()), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().mcg_scene_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def solve(self, prob, counts=False):
+        """FieldProblem::solve on the GPU.  Returns (field rows x cols, stats
+        dict, CountsBuffers or None)."""
+        info, _, _ = plan(self.mat, self.dom, prob)
+        out = np.zeros(info.rows * info.cols)
+        nb = prob.bundle_count if prob.bundle_count else info.nemit - prob.first_bundle
+        cb = abi.CountsBuffers(nb) if counts else None
+        st = abi.Stats()
+        check(lib().mcg_solve(self.handle, C.byref(prob), abi.dptr(out), out.size,
+                              C.byref(cb.struct) if cb else None, C.byref(st)))
+        return out.reshape(info.cols, info.rows).T.copy(), st.as_dict(), cb
+
+    def trace(self, prob, bundle, cap=4096):
+        """Step records of one bundle (mcg_trace): array (nsteps, 16)."""
+        rec = np.zeros(cap * TRACE_WIDTH)
+        n = C.c_int64()
+        check(lib().mcg_trace(self.handle, C.byref(prob), int(bundle), abi.dptr(rec), cap,
+                              C.byref(n)))
+        return rec.reshape(cap, TRACE_WIDTH)[:n.value].copy()
+

@@ -1,0 +1,148 @@
+"""Scene (domain) descriptions in the engine's C ABI layout.
+
+A :class:`Domain` collects convex cells, periodic/interface pairs and pores
+and produces the ``mcg_domain_spec`` both the engine and its checker read.
+Cells are described by the reference's constructor arguments: origin,
+edge matrix, tally divisions, imposed gradient and one (kind, T) per face in
+the reference's face order (subdomain.h:149-154 for the parallelepiped:
+Bac, Lef, Bot, Fro, Rig, Top).
+
+The workload builders below compose those cells the way the reference's
+domains do (BulkDomain / FilmDomain, domain.cpp:137-148, 188-199) and add
+the slab, bar and porous-film families the workloads name.
+"""
+import numpy as np
+
+from . import abi
+
+BAC, LEF, BOT, FRO, RIG, TOP = range(6)   # parallelepiped faces
+
+
+def face(kind, temp=0.0, specularity=0.0):
+    return (int(kind), float(temp), float(specularity))
+
+
+class Domain:
+    def __init__(self, name="domain"):
+        self.name = name
+        self.cells = []
+        self.pairs = []
+        self.pores = []
+
+    # -- construction -------------------------------------------------------
+    def add_cell(self, cell, origin, mat_cols, div, faces, grad_t=(0.0, 0.0, 0.0)):
+        """Add a convex cell; ``mat_cols`` is a 3x3 array whose *columns* are
+        the edge vectors (the reference's ``mat``).  Returns its index."""
+        nface = abi.NFACE[cell]
+        if len(faces) != nface:
+            raise ValueError(f"cell kind {cell} needs {nface} faces")
+        s = abi.SubdomainSpec()
+        s.cell = cell
+        s.nface = nface
+        m = np.asarray(mat_cols, dtype=np.float64)
+        s.origin[:] = [float(v) for v in origin]
+        s.mat[:] = [float(v) for v in m.T.reshape(-1)]    # column-major
+        s.div[:] = [int(v) for v in div]
+        s.grad_t[:] = [float(v) for v in grad_t]
+        for i, (k, t, p) in enumerate(faces):
+            s.face[i].kind = k
+            s.face[i].temp = t
+            s.face[i].specularity = p
+        self.cells.append(s)
+        return len(self.cells) - 1
+
+    def add_box(self, origin, dims, div, faces, grad_t=(0.0, 0.0, 0.0)):
+        return self.add_cell(abi.PARALLELEPIPED, origin, np.diag(dims), div, faces, grad_t)
+
+    def pair(self, sdom_a, face_a, sdom_b, face_b, transl=(0.0, 0.0, 0.0), rot=None):
+        """makePair: periodic (with translation/rotation) or interface."""
+        p = abi.PairSpec()
+        p.sdom_a, p.face_a, p.sdom_b, p.face_b = sdom_a, face_a, sdom_b, face_b
+        p.transl[:] = [float(v) for v in transl]
+        if rot is not None:
+            p.rot[:] = [float(v) for v in np.asarray(rot, dtype=np.float64).T.reshape(-1)]
+        self.pairs.append(p)
+
+    def add_pore(self, sdom, center, radius, kind=abi.DIFF, specularity=0.0):
+        p = abi.PoreSpec()
+        p.sdom, p.kind = sdom, kind
+        p.center[:] = [float(center[0]), float(center[1])]
+        p.radius = float(radius)
+        p.specularity = float(specularity)
+        self.pores.append(p)
+
+    # -- ABI ----------------------------------------------------------------
+    def spec(self):
+        """An ``mcg_domain_spec`` pointing into arrays owned by this object."""
+        self._sarr = (abi.SubdomainSpec * len(self.cells))(*self.cells)
+        self._parr = (abi.PairSpec * max(1, len(self.pairs)))(*self.pairs)
+        self._oarr = (abi.PoreSpec * max(1, len(self.pores)))(*self.pores)
+        d = abi.DomainSpec()
+        d.nsdom = len(self.cells)
+        d.npair = len(self.pairs)
+        d.npore = len(self.pores)
+        d.sdom = self._sarr
+        d.pair = self._parr
+        d.pore = self._oarr
+        self._spec = d
+        return d
+
+
+def slab(thickness, div, t_hot=1.0, t_cold=-1.0, side=abi.SPEC, specularity=0.0,
+         lateral=None):
+    """Cross-plane slab: isothermal walls at x = 0 (t_hot) and x = L (t_cold),
+    lateral walls of kind ``side`` (specular by default).  The lateral size
+    defaults to the thickness."""
+    lat = thickness if lateral is None else lateral
+    sf = face(side, 0.0, specularity)
+    d = Domain("slab")
+    d.add_box((0, 0, 0), (thickness, lat, lat), (div, 0, 0),
+              [face(abi.ISOT, t_hot), sf, sf, face(abi.ISOT, t_cold), sf, sf])
+    return d
+
+
+def bar(length, width, height, div, t_hot=10.0, t_cold=-10.0, side=abi.DIFF):
+    """Rectangular bar along x with isothermal ends and diffuse side walls."""
+    sf = face(side)
+    d = Domain("bar")
+    d.add_box((0, 0, 0), (length, width, height), (div, 0, 0),
+              [face(abi.ISOT, t_hot), sf, sf, face(abi.ISOT, t_cold), sf, sf])
+    return d
+
+
+def bulk(dim, div, dT):
+    """BulkDomain (domain.cpp:137-148): periodic box along x with a volumetric
+    gradient source -dT/dim[0], specular lateral walls."""
+    d = Domain("bulk")
+    s = d.add_box((0, 0, 0), dim, div,
+                  [face(abi.PERI), face(abi.SPEC), face(abi.SPEC),
+                   face(abi.PERI), face(abi.SPEC), face(abi.SPEC)],
+                  grad_t=(-dT / dim[0], 0.0, 0.0))
+    d.pair(s, BAC, s, FRO, transl=(dim[0], 0.0, 0.0))
+    return d
+
+
+def film(dim, div, dT):
+    """FilmDomain (domain.cpp:188-199): as bulk with diffuse Lef/Rig walls."""
+    d = Domain("film")
+    s = d.add_box((0, 0, 0), dim, div,
+                  [face(abi.PERI), face(abi.DIFF), face(abi.SPEC),
+                   face(abi.PERI), face(abi.DIFF), face(abi.SPEC)],
+                  grad_t=(-dT / dim[0], 0.0, 0.0))
+    d.pair(s, BAC, s, FRO, transl=(dim[0], 0.0, 0.0))
+    return d
+
+
+def problem(nemit, tally=abi.MULTI, maxscat=10 ** 9, maxloop=0, seed=1, precision=abi.FP64,
+            size=0, first=0, count=0):
+    p = abi.ProblemSpec()
+    p.tally = tally
+    p.precision = precision
+    p.nemit = int(nemit)
+    p.maxscat = int(maxscat)
+    p.maxloop = int(maxloop)
+    p.size = int(size)
+    p.seed = int(seed)
+    p.first_bundle = int(first)
+    p.bundle_count = int(count)
+    return p

@@ -1,0 +1,118 @@
+"""Deterministic-mode parity of the HIP engine against the FP64 CPU oracle.
+
+Both run the same Philox stream per bundle, so in FP64 every bundle's
+scatter count, boundary-hit count, step count and fate must be identical,
+and the normalised tallies must agree within 1e-5 (row-norm relative; the
+only differences are the order of FP64 additions).
+"""
+import numpy as np
+import pytest
+
+from montecarlocpp_amd import abi, engine, materials, scenes
+from oracle import oracle
+from tests.helpers import assert_counts_equal, rownorm_rel
+
+pytestmark = pytest.mark.gpu
+
+TALLY_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def si300():
+    return materials.si_like(300.0)
+
+
+def run_both(mat, dom, prob):
+    print(f"case: {dom.name} tally={prob.tally} prec={prob.precision} nemit={prob.nemit}", flush=True)
+    with engine.Scene(mat, dom) as sc:
+        fg, sg, cg = sc.solve(prob, counts=True)
+    fo, so, co = oracle.solve(mat, dom, prob, counts=True)
+    return fg, sg, cg, fo, so, co
+
+
+def check_det(fg, sg, cg, fo, so, co):
+    assert_counts_equal(cg, co)
+    for k in ("bundles", "steps", "escapes", "absorbed", "truncated"):
+        assert sg[k] == so[k], k
+    err = rownorm_rel(fg, fo)
+    assert np.all(err < TALLY_TOL), err
+
+
+def test_slab_workload_fp64(si300):
+    """1-D cross-plane slab, 100 nm, isothermal walls +/-1 K, 1e5 bundles."""
+    dom = scenes.slab(100e-9, 50)
+    r = run_both(si300, dom, scenes.problem(100_000, seed=1))
+    check_det(*r)
+
+
+def test_bar_fp64(si300):
+    """3-D bar 1 um x 100 nm x 100 nm with diffuse side walls."""
+    dom = scenes.bar(1e-6, 1e-7, 1e-7, 100)
+    r = run_both(si300, dom, scenes.problem(200_000, seed=7))
+    check_det(*r)
+
+
+@pytest.mark.parametrize("p", [0.0, 0.35, 1.0])
+def test_partial_specularity_fp64(si300, p):
+    dom = scenes.slab(300e-9, 60, side=abi.PARTIAL, specularity=p)
+    r = run_both(si300, dom, scenes.problem(50_000, seed=11))
+    check_det(*r)
+
+
+@pytest.mark.parametrize("tally", [abi.TEMP, abi.FLUX, abi.MULTI])
+def test_tally_kinds_fp64(si300, tally):
+    dom = scenes.bar(4e-7, 1e-7, 5e-8, 40)
+    r = run_both(si300, dom, scenes.problem(40_000, tally=tally, seed=5))
+    check_det(*r)
+
+
+@pytest.mark.parametrize("tally", [abi.CUMTEMP, abi.CUMFLUX])
+def test_cumulative_kinds_fp64(si300, tally):
+    dom = scenes.film((2e-7, 5e-8, 2e-7), (20, 0, 0), 1.0)
+    r = run_both(si300, dom, scenes.problem(30_000, tally=tally, maxscat=41, size=8, seed=9))
+    check_det(*r)
+
+
+def test_bulk_periodic_volume_emitter(si300):
+    """BulkDomain: periodic along x with the volumetric gradient emitter."""
+    dom = scenes.bulk((2e-7, 1e-7, 1e-7), (16, 0, 0), 0.2)
+    r = run_both(si300, dom, scenes.problem(30_000, maxscat=25, seed=3))
+    check_det(*r)
+
+
+def test_film_periodic(si300):
+    dom = scenes.film((2e-7, 6e-8, 2e-7), (0, 12, 0), 0.2)
+    r = run_both(si300, dom, scenes.problem(30_000, maxscat=30, seed=4))
+    check_det(*r)
+
+
+@pytest.mark.parametrize("div", [(12, 8, 0), (5, 4, 6)])
+def test_grid_2d_3d_fp64(si300, div):
+    """2-D / 3-D maps: the crossing walk must split segments as the ordered
+    crossing map of the reference does."""
+    d = scenes.Domain("box2d3d")
+    f = scenes.face
+    d.add_box((0, 0, 0), (3e-7, 2e-7, 1.5e-7), div,
+              [f(abi.ISOT, 1.0), f(abi.DIFF), f(abi.SPEC), f(abi.ISOT, -1.0), f(abi.DIFF),
+               f(abi.SPEC)])
+    r = run_both(si300, d, scenes.problem(30_000, seed=21))
+    check_det(*r)
+
+
+def test_maxloop_truncation(si300):
+    dom = scenes.film((2e-7, 6e-8, 2e-7), (10, 0, 0), 0.2)
+    r = run_both(si300, dom, scenes.problem(20_000, maxscat=1000, maxloop=7, seed=8))
+    check_det(*r)
+    assert r[1]["truncated"] > 0
+
+
+def test_bundle_subrange(si300):
+    """A sub-range of bundles gives the same counts as the full run."""
+    dom = scenes.slab(100e-9, 20)
+    full = scenes.problem(20_000, seed=2)
+    part = scenes.problem(20_000, seed=2, first=5000, count=7000)
+    with engine.Scene(si300, dom) as sc:
+        _, _, cf = sc.solve(full, counts=True)
+        _, _, cp = sc.solve(part, counts=True)
+    assert np.array_equal(cf.nscat[5000:12000], cp.nscat)
+    assert np.array_equal(cf.nstep[5000:12000], cp.nstep)
