@@ -1,0 +1,153 @@
+"""Benchmark of the phonon trajectory engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the 3-D bar the headline metric is quoted on): a 1 um x 100 nm x
+100 nm bar of the Si-like three-branch material at T_ref = 300 K, ends
+isothermal at +10 K / -10 K, four diffuse side walls, 1e7 energy bundles,
+100 x 1 x 1 Multi tally (temperature + heat flux), FP32 flight with FP64
+tallies and sampling tables.  One timed step is one complete field solve
+(mcg_solve) of all 1e7 bundles with the scene already resident on the GPU.
+
+Printed: one JSON line with the scatter-step throughput, the roofline
+object of the trajectory kernel (algorithmic bytes = 96 B of bundle state
+round trip per scatter-step), the tally error of the FP32-flight run against
+the FP64 run of the same streams, and the FP64 CPU checker timed on a
+bounded sample of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from montecarlocpp_amd import abi, engine, materials, scenes  # noqa: E402
+
+METRIC = "phonon scatter-steps/s at ≤1% tally error; p50/p99 solve latency (serve)"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_STEP_FP32 = 96   # 48 B SoA bundle state read + written per scatter-step
+BYTES_PER_STEP_FP64 = 152  # 76 B state in FP64
+NBUNDLES = 10_000_000
+CPU_SAMPLE = 2_000_000
+
+
+def workload():
+    mat = materials.si_like(300.0)
+    dom = scenes.bar(1e-6, 1e-7, 1e-7, 100, t_hot=10.0, t_cold=-10.0)
+    return mat, dom
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the solve shards by "
+              "bundle range across GPUs, which this build does not run.", file=sys.stderr)
+        sys.exit(2)
+
+    mat, dom = workload()
+    seed = 1
+    with engine.Scene(mat, dom) as sc:
+        prob = scenes.problem(NBUNDLES, tally=abi.MULTI, seed=seed, precision=abi.FP32_FLIGHT)
+        for w in range(args.warmup):
+            sc.solve(scenes.problem(NBUNDLES, seed=1000 + w, precision=abi.FP32_FLIGHT))
+        steps = 0
+        kern_ms = []
+        fields = []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            f, st, _ = sc.solve(scenes.problem(NBUNDLES, seed=seed + k, precision=abi.FP32_FLIGHT))
+            steps += st["steps"]
+            kern_ms.append(st["kernel_ms"])
+            fields.append(f)
+        elapsed = time.perf_counter() - t0
+        # precision split: the same streams in FP64
+        f64, st64, _ = sc.solve(scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64))
+
+    value = steps / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+    mean_kern_s = float(np.mean(kern_ms)) * 1e-3
+    steps_per_launch = steps / args.steps
+    achieved = steps_per_launch * BYTES_PER_STEP_FP32 / mean_kern_s
+
+    # tally error: FP32 flight against FP64 on the same seed, and the
+    # statistical standard error of the K independent solves
+    f32 = fields[0]
+    q = np.array([f[1].mean() for f in fields])
+    Tprof = np.array([f[0] for f in fields])
+    tally_error = {
+        "heat_flux_fp32_vs_fp64": abs(f32[1].mean() - f64[1].mean()) / abs(f64[1].mean()),
+        "temperature_fp32_vs_fp64": rel(f32[0], f64[0]),
+        "heat_flux_stat_rel_se": float(q.std(ddof=1) / np.sqrt(len(q)) / abs(q.mean())),
+        "temperature_stat_rel_se": float(np.linalg.norm(Tprof.std(axis=0, ddof=1))
+                                         / np.sqrt(len(q)) / np.linalg.norm(Tprof.mean(axis=0))),
+        "k_eff_W_per_mK": float(q.mean() * 1e-6 / 20.0),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle  # CPU baseline leg only
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        cprob = scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64, count=CPU_SAMPLE)
+        t1 = time.perf_counter()
+        _, cst, _ = oracle.solve(mat, dom, cprob, threads=threads)
+        ct = time.perf_counter() - t1
+        cpu = {"value": cst["steps"] / ct, "unit": "scatter-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {CPU_SAMPLE} of the {NBUNDLES} bundles of the same bar "
+                         f"workload, FP64, OpenMP static schedule over bundles "
+                         f"({cst['steps']} scatter-steps in {ct:.2f} s)"}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "scatter-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "bar 1um x 100nm x 100nm, ends +/-10 K, diffuse sides, Si-like "
+                        "3-branch material at 300 K, 1e7 bundles, 100x1x1 Multi tally, "
+                        "FP32 flight + FP64 tallies (one step = one full solve)",
+            "bundles": NBUNDLES,
+            "scatter_steps_per_solve": steps_per_launch,
+            "fp64_scatter_steps_per_s": st64["steps"] / (st64["kernel_ms"] * 1e-3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "kernel": "trajectory_kernel<float, true, true>",
+            "kernel_ms_mean": mean_kern_s * 1e3,
+            "bytes_per_step": BYTES_PER_STEP_FP32,
+        },
+        "tally_error": tally_error,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
