@@ -1405,6 +1405,19 @@ int oracle_sample_emission(const mcg_material_spec* mat, const mcg_domain_spec* 
     return MCG_OK;
 }
 
+// The portable math functions, element-wise (for accuracy tests).
+void oracle_math(int32_t fn, const double* x, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (fn == 0) {
+            out[i] = mcg_math::log(x[i]);
+        } else {
+            double s, c;
+            mcg_math::sincospi(x[i], &s, &c);
+            out[i] = fn == 1 ? s : c;
+        }
+    }
+}
+
 // Step-by-step record of one bundle (see mcg_trace in include/mcg.h).
 int oracle_trace(const mcg_material_spec* mat, const mcg_domain_spec* dom,
                  const mcg_problem_spec* prob, int64_t bundle, double* rec, int64_t cap,

@@ -56,6 +56,9 @@ def lib():
         L.oracle_trace.argtypes = [C.POINTER(abi.MaterialSpec), C.POINTER(abi.DomainSpec),
                                    C.POINTER(abi.ProblemSpec), C.c_int64, C.POINTER(C.c_double),
                                    C.c_int64, C.POINTER(C.c_int64)]
+        L.oracle_math.argtypes = [C.c_int32, C.POINTER(C.c_double), C.c_int64,
+                                  C.POINTER(C.c_double)]
+        L.oracle_math.restype = None
         _lib = L
     return _lib
 
@@ -157,4 +160,12 @@ def trace(mat, dom, prob, bundle, cap=4096):
     _check(lib().oracle_trace(C.byref(mat.spec), C.byref(dom.spec()), C.byref(prob), int(bundle),
                               abi.dptr(rec), cap, C.byref(n)))
     return rec.reshape(cap, 16)[:n.value].copy()
+
+
+def math_eval(fn, x):
+    """fn 0: log, 1: sin(pi x), 2: cos(pi x) with the portable FP64 functions."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    lib().oracle_math(fn, abi.dptr(x), x.size, abi.dptr(out))
+    return out
 

@@ -1,0 +1,81 @@
+"""The C ABI library: it loads without a GPU, exports every entry point the
+headers in include/ declare, and its host-only entry points work on the CPU.
+GPU entry points are exercised by the gpu-marked tests."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from montecarlocpp_amd import abi, engine, materials, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            text = open(os.path.join(inc, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names.update(re.findall(r"\b(mcg_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.lib()
+    names = declared()
+    assert {"mcg_solve", "mcg_scene_create", "mcg_trace", "mcg_batch_solve"} <= names
+    for n in sorted(names):
+        assert hasattr(lib, n), f"{n} declared in include/ but not exported"
+    assert set(engine.EXPORTS) == names
+    out = subprocess.run(["nm", "-D", "--defined-only", engine.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (mcg_[a-z0-9_]+)\b", out))
+    assert names <= exported
+
+
+def test_abi_version():
+    assert engine.lib().mcg_abi_version() == 1
+
+
+def test_host_entry_points_without_gpu():
+    mat = materials.si_like(300.0)
+    t = engine.material_tables(mat)
+    assert t["nmodes"] == 3000
+    info, w, n = engine.plan(mat, scenes.slab(1e-7, 50), scenes.problem(1000))
+    assert info.nemit == 1000 and info.rows == 4 and info.cols == 50
+
+
+def test_errors_are_status_codes():
+    lib = engine.lib()
+    assert lib.mcg_solve(None, None, None, 0, None, None) == abi.MCG_EINVAL
+    assert b"NULL" in lib.mcg_last_error()
+    mat = materials.si_like(300.0)
+    with pytest.raises(engine.EngineError) as e:
+        engine.plan(mat, scenes.slab(1e-7, 50), scenes.problem(0))
+    assert e.value.code == abi.MCG_EINVAL
+
+
+def test_scene_create_without_gpu_fails_cleanly():
+    """With no GPU visible, creating a scene returns MCG_EDEVICE (run in a
+    child process so that this test never touches a real device)."""
+    code = r"""
+import sys, ctypes as C
+sys.path.insert(0, %r)
+from montecarlocpp_amd import engine, materials, scenes, abi
+mat = materials.si_like(300.0); dom = scenes.slab(1e-7, 10)
+h = C.c_void_p()
+rc = engine.lib().mcg_scene_create(C.byref(mat.spec), C.byref(dom.spec()), C.byref(h))
+print(rc, engine.lib().mcg_last_error().decode())
+""" % ROOT
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rc = int(r.stdout.split()[0])
+    assert rc == abi.MCG_EDEVICE
