@@ -1,0 +1,806 @@
+// montecarlocpp.hpp -- the reference's C++ solve API, routed to the GPU.
+//
+// Header-only C++ mirror of the classes a montecarlocpp caller uses to set up
+// and run a field solve: Material (material.h:22-70), the boundary family
+// (boundary.h:33-281), the cell templates (subdomain.h:35-352), Domain with
+// BulkDomain / FilmDomain (domain.h:37-137), Field (field.h:26-49),
+// Statistics (field.h:51-66), Problem / FieldProblem and the five tally kinds
+// (problem.h:65-239), Progress (problem.h:42-60) and solveField / calcStats
+// (main.cpp:146-210).  Names, constructor arguments and their meaning follow
+// the reference, so a deck written against it compiles against this header;
+// FieldProblem::solve hands the whole solve to the MI355X engine through the
+// C ABI of include/mcg.h (one mcg_solve per solve, not per OpenMP thread).
+//
+// Eigen and Boost are not used; the small Vector3d / Matrix3d / Vector3l /
+// ArrayXXd types below cover what the API passes around.  Errors that the
+// reference asserts on are thrown as mcg::Error with the same message.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "mcg.h"
+
+namespace mcg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc) {
+    if (rc != MCG_OK) throw Error(rc, mcg_last_error());
+}
+
+}  // namespace mcg
+
+// --------------------------------------------------------------- small algebra
+struct Vector3d {
+    double v[3] = {0, 0, 0};
+    Vector3d() = default;
+    Vector3d(double x, double y, double z) : v{x, y, z} {}
+    double& operator()(int i) { return v[i]; }
+    double operator()(int i) const { return v[i]; }
+    static Vector3d Zero() { return Vector3d(); }
+    static Vector3d Unit(int i) { Vector3d r; r.v[i] = 1.0; return r; }
+    Vector3d operator+(const Vector3d& o) const { return {v[0] + o.v[0], v[1] + o.v[1], v[2] + o.v[2]}; }
+    Vector3d operator-(const Vector3d& o) const { return {v[0] - o.v[0], v[1] - o.v[1], v[2] - o.v[2]}; }
+    Vector3d operator*(double s) const { return {v[0] * s, v[1] * s, v[2] * s}; }
+};
+inline Vector3d operator*(double s, const Vector3d& a) { return a * s; }
+
+struct Vector3l {
+    long v[3] = {0, 0, 0};
+    Vector3l() = default;
+    Vector3l(long x, long y, long z) : v{x, y, z} {}
+    long& operator()(int i) { return v[i]; }
+    long operator()(int i) const { return v[i]; }
+};
+
+// 3x3, column-major like Eigen's Matrix3d
+struct Matrix3d {
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double& operator()(int r, int c) { return m[3 * c + r]; }
+    double operator()(int r, int c) const { return m[3 * c + r]; }
+    static Matrix3d Identity() { Matrix3d a; a.m[0] = a.m[4] = a.m[8] = 1.0; return a; }
+    static Matrix3d Zero() { return Matrix3d(); }
+    static Matrix3d Diagonal(const Vector3d& d) {
+        Matrix3d a;
+        a.m[0] = d(0); a.m[4] = d(1); a.m[8] = d(2);
+        return a;
+    }
+    Vector3d col(int c) const { return {m[3 * c], m[3 * c + 1], m[3 * c + 2]}; }
+};
+
+using VectorXd = std::vector<double>;
+
+// Column-major rows x cols array (Eigen's ArrayXXd layout).
+class ArrayXXd {
+    long rows_ = 0, cols_ = 0;
+    std::vector<double> d_;
+
+public:
+    ArrayXXd() = default;
+    ArrayXXd(long r, long c) : rows_(r), cols_(c), d_(size_t(r * c), 0.0) {}
+    static ArrayXXd Zero(long r, long c) { return ArrayXXd(r, c); }
+    long rows() const { return rows_; }
+    long cols() const { return cols_; }
+    long size() const { return rows_ * cols_; }
+    double* data() { return d_.data(); }
+    const double* data() const { return d_.data(); }
+    double& operator()(long r, long c) { return d_[size_t(c * rows_ + r)]; }
+    double operator()(long r, long c) const { return d_[size_t(c * rows_ + r)]; }
+    ArrayXXd& operator+=(const ArrayXXd& o) {
+        for (size_t i = 0; i < d_.size(); ++i) d_[i] += o.d_[i];
+        return *this;
+    }
+    ArrayXXd operator-(const ArrayXXd& o) const { ArrayXXd r(*this); for (size_t i = 0; i < d_.size(); ++i) r.d_[i] -= o.d_[i]; return r; }
+    ArrayXXd operator*(const ArrayXXd& o) const { ArrayXXd r(*this); for (size_t i = 0; i < d_.size(); ++i) r.d_[i] *= o.d_[i]; return r; }
+    ArrayXXd operator/(double s) const { ArrayXXd r(*this); for (double& x : r.d_) x /= s; return r; }
+    ArrayXXd operator+(const ArrayXXd& o) const { ArrayXXd r(*this); r += o; return r; }
+    ArrayXXd sqrt() const { ArrayXXd r(*this); for (double& x : r.d_) x = std::sqrt(x); return r; }
+};
+
+// ------------------------------------------------------------------ Rng
+// The reference's Rng is a Boost mt19937 (random.h:22) seeded per thread.
+// Here it only carries the seed of the solve's Philox key: the engine derives
+// every bundle's stream from (seed, bundle id).
+class Rng {
+    std::mt19937_64 g_;
+
+public:
+    explicit Rng(uint64_t seed = 5489u) : g_(seed) {}
+    uint64_t operator()() { return g_(); }
+};
+
+// --------------------------------------------------------------- Material
+// Material(disp, relax, T): the reference's two text formats
+// (material.cpp:87-114), parsed here; the tables are built by the engine.
+class Material {
+    long nw_ = 0, np_ = 0;
+    double T_ = 0;
+    std::vector<double> disp_, relax_;
+    mcg_material_tables tab_{};
+    std::string dispName_, relaxName_;
+
+    void compute() {
+        mcg_material_spec s = spec();
+        mcg::check(mcg_material_compute(&s, &tab_));
+    }
+
+public:
+    Material() = default;
+    Material(const std::string& disp, const std::string& relax, double temp)
+        : T_(temp), dispName_(disp), relaxName_(relax) {
+        std::ifstream df(disp.c_str());
+        if (!df) throw mcg::Error(MCG_EINVAL, "Error opening dispersion file");
+        df >> nw_ >> np_;
+        if (!(nw_ > 0 && np_ > 0)) throw mcg::Error(MCG_EINVAL, "Invalid dispersion file");
+        disp_.resize(size_t(nw_ * (2 + 2 * np_)));
+        for (double& x : disp_)
+            if (!(df >> x)) throw mcg::Error(MCG_EINVAL, "Array extraction failed");
+        std::ifstream rf(relax.c_str());
+        if (!rf) throw mcg::Error(MCG_EINVAL, "Error opening relaxation time file");
+        relax_.resize(size_t(8 * np_));
+        for (double& x : relax_)
+            if (!(rf >> x)) throw mcg::Error(MCG_EINVAL, "Array extraction failed");
+        compute();
+    }
+    Material(long nw, long np, std::vector<double> disp, std::vector<double> relax, double temp)
+        : nw_(nw), np_(np), T_(temp), disp_(std::move(disp)), relax_(std::move(relax)) {
+        compute();
+    }
+    mcg_material_spec spec() const {
+        return mcg_material_spec{nw_, np_, disp_.data(), relax_.data(), T_};
+    }
+    double temp() const { return T_; }
+    double cond() const { return tab_.cond; }
+    double energySum() const { return tab_.energy_sum; }
+    double fluxSum() const { return tab_.flux_sum; }
+    double scatSum() const { return tab_.scat_sum; }
+    friend std::ostream& operator<<(std::ostream& os, const Material& m) {
+        return os << "Material " << &m << "\n  disp:  " << m.dispName_ << "\n  relax: "
+                  << m.relaxName_ << "\n  temp:  " << m.T_;
+    }
+};
+
+// --------------------------------------------------------------- Phonon
+// The bundle state (phonon.h:20-71); on the GPU it lives in registers.  This
+// host type is what a trajectory record describes.
+class Phonon {
+public:
+    class Prop {
+        long w_ = 0, p_ = 0;
+
+    public:
+        Prop() = default;
+        Prop(long w, long p) : w_(w), p_(p) {}
+        long w() const { return w_; }
+        long p() const { return p_; }
+    };
+    bool alive = true;
+    int sign = 1;
+    Prop prop;
+    Vector3d pos, dir;
+    double scatNext = 0;
+    long nscat = 0;
+};
+
+// ------------------------------------------------------------- boundaries
+class Subdomain;
+
+class Boundary {
+public:
+    class Shape {
+    public:
+        virtual ~Shape() = default;
+    };
+    virtual ~Boundary() = default;
+    virtual int kind() const = 0;
+    virtual std::string type() const = 0;
+    double temp() const { return T_; }
+    double specularity() const { return p_; }
+
+protected:
+    double T_ = 0.0;
+    double p_ = 0.0;
+    friend class Subdomain;
+};
+
+class Parallelogram : public Boundary::Shape {
+    Vector3d i_, j_;
+
+public:
+    Parallelogram() = default;
+    Parallelogram(const Vector3d& i, const Vector3d& j) : i_(i), j_(j) {}
+};
+class Triangle : public Boundary::Shape {
+    Vector3d i_, j_;
+
+public:
+    Triangle() = default;
+    Triangle(const Vector3d& i, const Vector3d& j) : i_(i), j_(j) {}
+};
+
+// Non-emitting faces: a temperature is an error (boundary.cpp:272-326).
+template <int K>
+class PlainBoundary : public Boundary {
+public:
+    PlainBoundary() = default;
+    template <typename S>
+    PlainBoundary(const Vector3d&, const S&, double T) {
+        if (T != 0.0) throw mcg::Error(MCG_EINVAL, "Cannot specify temperature");
+    }
+    int kind() const override { return K; }
+    std::string type() const override {
+        return K == MCG_SPEC ? "Spec" : K == MCG_DIFF ? "Diff" : "Inter";
+    }
+};
+using SpecBoundary = PlainBoundary<MCG_SPEC>;
+using DiffBoundary = PlainBoundary<MCG_DIFF>;
+
+class InterBoundary : public PlainBoundary<MCG_INTER> {
+public:
+    using PlainBoundary<MCG_INTER>::PlainBoundary;
+    std::vector<InterBoundary*> pairs;      // makePair partners added by this face
+};
+
+// Extension: specular with probability p, else diffuse.
+class PartialBoundary : public Boundary {
+public:
+    PartialBoundary() = default;
+    template <typename S>
+    PartialBoundary(const Vector3d&, const S&, double T) {
+        if (T != 0.0) throw mcg::Error(MCG_EINVAL, "Cannot specify temperature");
+    }
+    void specularity(double p) { p_ = p; }
+    double specularity() const { return p_; }
+    int kind() const override { return MCG_PARTIAL; }
+    std::string type() const override { return "Partial"; }
+};
+
+template <typename S>
+class IsotBoundary : public Boundary {
+public:
+    IsotBoundary() = default;
+    IsotBoundary(const Vector3d&, const S&, double T) { T_ = T; }
+    int kind() const override { return MCG_ISOT; }
+    std::string type() const override { return "Isot"; }
+};
+
+// The periodic map of a makePair call, kept on its first face.
+class PeriBase : public Boundary {
+public:
+    int kind() const override { return MCG_PERI; }
+    std::string type() const override { return "Peri"; }
+    const PeriBase* pair = nullptr;
+    Vector3d transl;
+    Matrix3d rot = Matrix3d::Identity();
+};
+
+template <typename S>
+class PeriBoundary : public PeriBase {
+public:
+    PeriBoundary() = default;
+    PeriBoundary(const Vector3d&, const S&, double T) { T_ = T; }
+};
+
+template <typename S>
+void makePair(PeriBoundary<S>& b1, PeriBoundary<S>& b2, const Vector3d& t,
+              const Matrix3d& r = Matrix3d::Identity()) {
+    if (b1.pair || b2.pair) throw mcg::Error(MCG_EINVAL, "Boundary already paired");
+    b1.pair = &b2;
+    b1.transl = t;
+    b1.rot = r;
+}
+inline void makePair(InterBoundary& b1, InterBoundary& b2) { b1.pairs.push_back(&b2); }
+
+// --------------------------------------------------------------- cells
+class Subdomain {
+public:
+    using Pointers = std::vector<const Subdomain*>;
+    virtual ~Subdomain() = default;
+    virtual std::vector<const Boundary*> bdryPtrs() const = 0;
+    mcg_subdomain_spec spec() const {
+        mcg_subdomain_spec s{};
+        s.cell = cell_;
+        for (int k = 0; k < 3; ++k) {
+            s.origin[k] = o_(k);
+            s.div[k] = div_(k);
+            s.grad_t[k] = gradT_(k);
+        }
+        for (int i = 0; i < 9; ++i) s.mat[i] = mat_.m[i];
+        std::vector<const Boundary*> b = bdryPtrs();
+        s.nface = int32_t(b.size());
+        for (size_t f = 0; f < b.size(); ++f) {
+            s.face[f].kind = b[f]->kind();
+            s.face[f].temp = b[f]->T_;
+            s.face[f].specularity = b[f]->p_;
+        }
+        return s;
+    }
+
+protected:
+    int cell_ = MCG_PARALLELEPIPED;
+    Vector3d o_, gradT_;
+    Matrix3d mat_;
+    Vector3l div_;
+};
+using EmitSubdomain = Subdomain;
+
+// Parallelepiped<Bac, Lef, Bot, Fro, Rig, Top>(o, mat, div, gradT, T)
+// (subdomain.h:128-198): faces at o (Bac, Lef, Bot) and o + mat.col(k).
+template <typename Bac, typename Lef, typename Bot, typename Fro = Bac, typename Rig = Lef,
+          typename Top = Bot>
+class Parallelepiped : public Subdomain {
+    std::tuple<Bac, Lef, Bot, Fro, Rig, Top> b_;
+
+public:
+    Parallelepiped(const Vector3d& o, const Matrix3d& mat, const Vector3l& div,
+                   const Vector3d& gradT = Vector3d::Zero(),
+                   const std::vector<double>& T = std::vector<double>(6, 0.0))
+        : b_(Bac(o, Parallelogram(mat.col(1), mat.col(2)), T.at(0)),
+             Lef(o, Parallelogram(mat.col(2), mat.col(0)), T.at(1)),
+             Bot(o, Parallelogram(mat.col(0), mat.col(1)), T.at(2)),
+             Fro(o + mat.col(0), Parallelogram(mat.col(2), mat.col(1)), T.at(3)),
+             Rig(o + mat.col(1), Parallelogram(mat.col(0), mat.col(2)), T.at(4)),
+             Top(o + mat.col(2), Parallelogram(mat.col(1), mat.col(0)), T.at(5))) {
+        if (T.size() != 6) throw mcg::Error(MCG_EINVAL, "Incorrect number of temperatures");
+        cell_ = MCG_PARALLELEPIPED;
+        o_ = o;
+        mat_ = mat;
+        div_ = div;
+        gradT_ = gradT;
+    }
+    Parallelepiped(const Parallelepiped&) = delete;
+    template <int I>
+    typename std::tuple_element<I, std::tuple<Bac, Lef, Bot, Fro, Rig, Top>>::type& bdry() {
+        return std::get<I>(b_);
+    }
+    std::vector<const Boundary*> bdryPtrs() const override {
+        return {&std::get<0>(b_), &std::get<1>(b_), &std::get<2>(b_),
+                &std::get<3>(b_), &std::get<4>(b_), &std::get<5>(b_)};
+    }
+};
+
+// TriangularPrism<Bac, Lef, Bot, Dia, Top> (subdomain.h:206-277)
+template <typename Bac, typename Lef, typename Bot, typename Dia, typename Top = Bot>
+class TriangularPrism : public Subdomain {
+    std::tuple<Bac, Lef, Bot, Dia, Top> b_;
+
+public:
+    TriangularPrism(const Vector3d& o, const Matrix3d& mat, const Vector3l& div,
+                    const Vector3d& gradT = Vector3d::Zero(),
+                    const std::vector<double>& T = std::vector<double>(5, 0.0))
+        : b_(Bac(o, Parallelogram(mat.col(1), mat.col(2)), T.at(0)),
+             Lef(o, Parallelogram(mat.col(2), mat.col(0)), T.at(1)),
+             Bot(o, Triangle(mat.col(0), mat.col(1)), T.at(2)),
+             Dia(o + mat.col(0), Parallelogram(mat.col(2), mat.col(1) - mat.col(0)), T.at(3)),
+             Top(o + mat.col(2), Triangle(mat.col(1), mat.col(0)), T.at(4))) {
+        if (T.size() != 5) throw mcg::Error(MCG_EINVAL, "Incorrect number of temperatures");
+        cell_ = MCG_TRIPRISM;
+        o_ = o;
+        mat_ = mat;
+        div_ = div;
+        gradT_ = gradT;
+    }
+    TriangularPrism(const TriangularPrism&) = delete;
+    template <int I>
+    typename std::tuple_element<I, std::tuple<Bac, Lef, Bot, Dia, Top>>::type& bdry() {
+        return std::get<I>(b_);
+    }
+    std::vector<const Boundary*> bdryPtrs() const override {
+        return {&std::get<0>(b_), &std::get<1>(b_), &std::get<2>(b_), &std::get<3>(b_),
+                &std::get<4>(b_)};
+    }
+};
+
+// Tetrahedron<Bac, Lef, Bot, Dia> (subdomain.h:285-352)
+template <typename Bac, typename Lef, typename Bot, typename Dia>
+class Tetrahedron : public Subdomain {
+    std::tuple<Bac, Lef, Bot, Dia> b_;
+
+public:
+    Tetrahedron(const Vector3d& o, const Matrix3d& mat, const Vector3l& div,
+                const Vector3d& gradT = Vector3d::Zero(),
+                const std::vector<double>& T = std::vector<double>(4, 0.0))
+        : b_(Bac(o, Triangle(mat.col(1), mat.col(2)), T.at(0)),
+             Lef(o, Triangle(mat.col(2), mat.col(0)), T.at(1)),
+             Bot(o, Triangle(mat.col(0), mat.col(1)), T.at(2)),
+             Dia(o + mat.col(0), Triangle(mat.col(2) - mat.col(0), mat.col(1) - mat.col(0)), T.at(3))) {
+        if (T.size() != 4) throw mcg::Error(MCG_EINVAL, "Incorrect number of temperatures");
+        cell_ = MCG_TETRAHEDRON;
+        o_ = o;
+        mat_ = mat;
+        div_ = div;
+        gradT_ = gradT;
+    }
+    Tetrahedron(const Tetrahedron&) = delete;
+    template <int I>
+    typename std::tuple_element<I, std::tuple<Bac, Lef, Bot, Dia>>::type& bdry() {
+        return std::get<I>(b_);
+    }
+    std::vector<const Boundary*> bdryPtrs() const override {
+        return {&std::get<0>(b_), &std::get<1>(b_), &std::get<2>(b_), &std::get<3>(b_)};
+    }
+};
+
+// ----------------------------------------------------------------- Domain
+// Domain (domain.h:37-79): owns nothing here; derived domains own their cells
+// and register them with addSdom in the reference's order.
+class Domain {
+    Subdomain::Pointers sdomPtrs_;
+    std::vector<mcg_subdomain_spec> cells_;
+    std::vector<mcg_pair_spec> pairs_;
+    std::vector<mcg_pore_spec> pores_;
+    mcg_domain_spec spec_{};
+
+    int locate(const Boundary* b, int* face) const {
+        for (size_t s = 0; s < sdomPtrs_.size(); ++s) {
+            std::vector<const Boundary*> bp = sdomPtrs_[s]->bdryPtrs();
+            for (size_t f = 0; f < bp.size(); ++f)
+                if (bp[f] == b) {
+                    *face = int(f);
+                    return int(s);
+                }
+        }
+        throw mcg::Error(MCG_EINVAL, "paired face belongs to no subdomain of the domain");
+    }
+
+protected:
+    void addSdom(const Subdomain* s) { sdomPtrs_.push_back(s); }
+    void addPore(int sdom, double cx, double cy, double r, int kind = MCG_DIFF, double p = 0.0) {
+        pores_.push_back(mcg_pore_spec{sdom, kind, {cx, cy}, r, p});
+    }
+
+public:
+    virtual ~Domain() = default;
+    Domain() = default;
+    Domain(const Domain&) = delete;
+    virtual std::string info() const = 0;
+    const Subdomain::Pointers& sdomPtrs() const { return sdomPtrs_; }
+
+    // The scene in the engine's layout: cells in registration order, one pair
+    // spec per makePair call.
+    const mcg_domain_spec& spec() {
+        cells_.clear();
+        pairs_.clear();
+        for (const Subdomain* s : sdomPtrs_) cells_.push_back(s->spec());
+        for (size_t s = 0; s < sdomPtrs_.size(); ++s) {
+            std::vector<const Boundary*> bp = sdomPtrs_[s]->bdryPtrs();
+            for (size_t f = 0; f < bp.size(); ++f) {
+                std::vector<std::pair<const Boundary*, const Matrix3d*>> partners;
+                const Vector3d* t = nullptr;
+                if (bp[f]->kind() == MCG_PERI) {
+                    auto* pb = static_cast<const PeriBase*>(bp[f]);
+                    if (pb->pair) {
+                        partners.push_back({pb->pair, &pb->rot});
+                        t = &pb->transl;
+                    }
+                } else if (bp[f]->kind() == MCG_INTER) {
+                    for (const InterBoundary* q : static_cast<const InterBoundary*>(bp[f])->pairs)
+                        partners.push_back({q, nullptr});
+                }
+                for (auto& pr : partners) {
+                    mcg_pair_spec ps{};
+                    ps.sdom_a = int32_t(s);
+                    ps.face_a = int32_t(f);
+                    int fb = 0;
+                    ps.sdom_b = locate(pr.first, &fb);
+                    ps.face_b = fb;
+                    if (t)
+                        for (int k = 0; k < 3; ++k) ps.transl[k] = (*t)(k);
+                    if (pr.second)
+                        for (int i = 0; i < 9; ++i) ps.rot[i] = pr.second->m[i];
+                    pairs_.push_back(ps);
+                }
+            }
+        }
+        spec_.nsdom = int32_t(cells_.size());
+        spec_.npair = int32_t(pairs_.size());
+        spec_.npore = int32_t(pores_.size());
+        spec_.sdom = cells_.data();
+        spec_.pair = pairs_.data();
+        spec_.pore = pores_.data();
+        return spec_;
+    }
+    virtual ArrayXXd average(const ArrayXXd& data) const { return data; }
+    friend std::ostream& operator<<(std::ostream& os, const Domain& d) { return os << d.info(); }
+};
+
+using PeriP = PeriBoundary<Parallelogram>;
+using Spec = SpecBoundary;
+using Diff = DiffBoundary;
+using Inter = InterBoundary;
+using IsotP = IsotBoundary<Parallelogram>;
+
+// BulkDomain (domain.h:93-114, domain.cpp:137-148)
+class BulkDomain : public Domain {
+public:
+    using Sdom = Parallelepiped<PeriP, Spec, Spec>;
+
+private:
+    Vector3d dim_;
+    Vector3l div_;
+    double dT_;
+    Sdom sdom_;
+
+public:
+    BulkDomain(const Vector3d& dim, const Vector3l& div, double dT)
+        : dim_(dim), div_(div), dT_(dT),
+          sdom_(Vector3d::Zero(), Matrix3d::Diagonal(dim), div, Vector3d(-dT / dim(0), 0., 0.)) {
+        makePair(sdom_.bdry<0>(), sdom_.bdry<3>(), Vector3d(dim(0), 0., 0.));
+        addSdom(&sdom_);
+    }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "BulkDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2)
+           << "]\n  div: [" << div_(0) << " " << div_(1) << " " << div_(2) << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+};
+
+// FilmDomain (domain.h:116-137, domain.cpp:188-199)
+class FilmDomain : public Domain {
+public:
+    using Sdom = Parallelepiped<PeriP, Diff, Spec>;
+
+private:
+    Vector3d dim_;
+    Vector3l div_;
+    double dT_;
+    Sdom sdom_;
+
+public:
+    FilmDomain(const Vector3d& dim, const Vector3l& div, double dT)
+        : dim_(dim), div_(div), dT_(dT),
+          sdom_(Vector3d::Zero(), Matrix3d::Diagonal(dim), div, Vector3d(-dT / dim(0), 0., 0.)) {
+        makePair(sdom_.bdry<0>(), sdom_.bdry<3>(), Vector3d(dim(0), 0., 0.));
+        addSdom(&sdom_);
+    }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "FilmDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2)
+           << "]\n  div: [" << div_(0) << " " << div_(1) << " " << div_(2) << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+};
+
+// Cross-plane slab / bar with isothermal ends (the workloads' domains; the
+// reference ships none with IsotBoundary, the composition is API-legal).
+template <typename Side>
+class WallDomain : public Domain {
+public:
+    using Sdom = Parallelepiped<IsotP, Side, Side>;
+
+private:
+    Vector3d dim_;
+    Sdom sdom_;
+
+public:
+    WallDomain(const Vector3d& dim, const Vector3l& div, double Thot, double Tcold)
+        : dim_(dim),
+          sdom_(Vector3d::Zero(), Matrix3d::Diagonal(dim), div, Vector3d::Zero(),
+                std::vector<double>{Thot, 0., 0., Tcold, 0., 0.}) {
+        addSdom(&sdom_);
+    }
+    Sdom& sdom() { return sdom_; }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "WallDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2) << "]";
+        return ss.str();
+    }
+};
+using SlabDomain = WallDomain<Spec>;
+using BarDomain = WallDomain<Diff>;
+
+// ------------------------------------------------------------------ Field
+// Field(rows, dom) (field.h:26-49): the rows x cells tally layout, one block
+// of columns per subdomain with a grid, in registration order (field.cpp:25-45).
+class Field {
+    ArrayXXd data_;
+
+public:
+    Field(long rows, Domain* dom) {
+        const mcg_domain_spec& d = dom->spec();
+        long cols = 0;
+        for (int s = 0; s < d.nsdom; ++s) {
+            const int64_t* dv = d.sdom[s].div;
+            if (dv[0] < 0 || dv[1] < 0 || dv[2] < 0) continue;
+            cols += std::max<int64_t>(dv[0], 1) * std::max<int64_t>(dv[1], 1) * std::max<int64_t>(dv[2], 1);
+        }
+        data_ = ArrayXXd(rows, cols);
+    }
+    const ArrayXXd& data() const { return data_; }
+};
+
+// ------------------------------------------------------------- Progress
+class Progress {
+    long tot_ = 0, count_ = 0, esc_ = 0;
+
+public:
+    Progress() = default;
+    Progress(long tot, long) : tot_(tot) {}
+    long count() const { return count_; }
+    long esc() const { return esc_; }
+    long incrCount() { return ++count_; }
+    long incrEsc() { return ++esc_; }
+    void add(long bundles, long esc) {
+        count_ += bundles;
+        esc_ += esc;
+    }
+};
+
+// ----------------------------------------------------------- Statistics
+// Welford mean / sample variance (field.cpp:226-262)
+template <typename S>
+class Statistics {
+    long n_ = 0;
+    S z_, m_, s_;
+
+public:
+    Statistics() = default;
+    explicit Statistics(const S& zero) : z_(zero), m_(zero), s_(zero) {}
+    S mean() const { return m_; }
+    S variance() const { return n_ < 2 ? z_ : s_ / double(n_ - 1); }
+    void add(const S& x) {
+        n_++;
+        S d = x - m_;
+        m_ += d / double(n_);
+        s_ += d * (x - m_);
+    }
+};
+
+// ---------------------------------------------------------------- Problem
+class Problem {
+    const Material* mat_;
+    Domain* dom_;
+
+public:
+    Problem(const Material* mat, Domain* dom) : mat_(mat), dom_(dom) {}
+    virtual ~Problem() = default;
+    const Material* mat() const { return mat_; }
+    Domain* dom() const { return dom_; }
+    virtual Progress initProgress() const = 0;
+    virtual ArrayXXd solve(Rng& gen, Progress* prog) const = 0;
+};
+
+// FieldProblem(mat, dom, nemit, maxscat, maxloop) (problem.cpp:315-342).
+// The engine rounds the per-emitter counts and computes power_; solve() runs
+// one GPU solve and returns power_ * postProc(data) / cellVol.
+class FieldProblem : public Problem {
+    mutable mcg_scene* scene_ = nullptr;
+    int tally_;
+
+protected:
+    long nemit_, maxscat_, maxloop_, size_ = 0;
+    int precision_ = MCG_FP32_FLIGHT;
+    mutable mcg_problem_info info_{};
+    mutable mcg_stats last_{};
+    mutable uint64_t lastKey_ = 0;
+
+    mcg_problem_spec problemSpec(uint64_t seed) const {
+        mcg_problem_spec p{};
+        p.tally = tally_;
+        p.precision = precision_;
+        p.nemit = nemit_;
+        p.maxscat = maxscat_;
+        p.maxloop = maxloop_;
+        p.size = size_;
+        p.seed = seed;
+        return p;
+    }
+
+public:
+    FieldProblem(const Material* mat, Domain* dom, int tally, long nemit, long maxscat,
+                 long maxloop)
+        : Problem(mat, dom), tally_(tally), nemit_(nemit), maxscat_(maxscat), maxloop_(maxloop) {}
+    ~FieldProblem() override {
+        if (scene_) mcg_scene_destroy(scene_);
+    }
+    void precision(int p) { precision_ = p; }
+    const mcg_problem_info& plan() const {
+        mcg_material_spec ms = mat()->spec();
+        mcg_problem_spec ps = problemSpec(0);
+        mcg::check(mcg_problem_plan(&ms, &dom()->spec(), &ps, &info_, nullptr, nullptr, 0));
+        return info_;
+    }
+    long nemit() const { return plan().nemit; }
+    double power() const { return plan().power; }
+    Progress initProgress() const override { return Progress(plan().nemit, std::min(20l, plan().nemit)); }
+    ArrayXXd initSolution() const { return ArrayXXd(plan().rows, info_.cols); }
+    const mcg_stats& lastStats() const { return last_; }
+    uint64_t lastKey() const { return lastKey_; }     // Philox key of the last solve
+
+    ArrayXXd solve(Rng& gen, Progress* prog) const override {
+        const mcg_problem_info& info = plan();
+        if (!scene_) {
+            mcg_material_spec ms = mat()->spec();
+            mcg::check(mcg_scene_create(&ms, &dom()->spec(), &scene_));
+        }
+        ArrayXXd out(info.rows, info.cols);
+        lastKey_ = gen();
+        mcg_problem_spec ps = problemSpec(lastKey_);
+        mcg::check(mcg_solve(scene_, &ps, out.data(), out.size(), nullptr, &last_));
+        if (prog) prog->add(last_.bundles, last_.escapes);
+        return out;
+    }
+};
+
+struct TempProblem : FieldProblem {
+    TempProblem(const Material* m, Domain* d, long nemit, long maxscat, long maxloop = 0)
+        : FieldProblem(m, d, MCG_TEMP, nemit, maxscat, maxloop) {}
+};
+struct FluxProblem : FieldProblem {
+    FluxProblem(const Material* m, Domain* d, long nemit, long maxscat, long maxloop = 0)
+        : FieldProblem(m, d, MCG_FLUX, nemit, maxscat, maxloop) {}
+};
+struct MultiProblem : FieldProblem {
+    MultiProblem(const Material* m, Domain* d, long nemit, long maxscat, long maxloop = 0)
+        : FieldProblem(m, d, MCG_MULTI, nemit, maxscat, maxloop) {}
+};
+struct CumTempProblem : FieldProblem {
+    CumTempProblem(const Material* m, Domain* d, long nemit, long size, long maxscat, long maxloop = 0)
+        : FieldProblem(m, d, MCG_CUMTEMP, nemit, maxscat, maxloop) {
+        size_ = size;
+    }
+};
+struct CumFluxProblem : FieldProblem {
+    CumFluxProblem(const Material* m, Domain* d, long nemit, long size, long maxscat, long maxloop = 0)
+        : FieldProblem(m, d, MCG_CUMFLUX, nemit, maxscat, maxloop) {
+        size_ = size;
+    }
+};
+
+// ------------------------------------------------------------ driver glue
+// printSolution (main.cpp:67-84): "%16.9e " per entry, one row per line.
+inline void printSolution(const ArrayXXd& sol, std::ostream& os = std::cout) {
+    char buf[64];
+    for (long i = 0; i < sol.rows(); ++i) {
+        for (long j = 0; j < sol.cols(); ++j) {
+            std::snprintf(buf, sizeof buf, "%16.9e ", sol(i, j));
+            os << buf;
+        }
+        os << "\n";
+    }
+    os << "\n";
+}
+
+// solveField (main.cpp:146-179): one GPU solve (the reference sums one
+// partial per OpenMP thread; the engine already covers every bundle).
+inline ArrayXXd solveField(const FieldProblem* prob, Rng& gen, std::ostream* log = nullptr) {
+    Progress prog = prob->initProgress();
+    ArrayXXd sol = prob->solve(gen, &prog);
+    if (log) {
+        *log << "Output\n";
+        printSolution(sol, *log);
+    }
+    return prob->dom()->average(sol);
+}
+
+// solveField(prob, nsim) + calcStats (main.cpp:181-210)
+inline Statistics<ArrayXXd> calcStats(const FieldProblem* prob, long nsim, Rng& gen,
+                                      std::ostream* log = nullptr) {
+    Statistics<ArrayXXd> stats(prob->initSolution());
+    for (long i = 0; i < nsim; ++i) stats.add(solveField(prob, gen, log));
+    if (log) {
+        *log << "Mean\n";
+        printSolution(stats.mean(), *log);
+        *log << "Standard Deviation\n";
+        printSolution(stats.variance().sqrt(), *log);
+    }
+    return stats;
+}

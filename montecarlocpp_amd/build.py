@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmcg.so")
+DRIVER = os.path.join(HERE, "api", "mcg_main")
 ROOT = os.path.dirname(HERE)
 
 ARCH = "gfx950"
@@ -52,8 +53,12 @@ def build(verbose=False, save_temps=False):
     tmp = LIB + ".tmp"
     _run([hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp, kern_obj, host_obj])
     os.replace(tmp, LIB)
+    # the reference-compatible command-line driver over the C ABI
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I",
+          os.path.join(HERE, "api"), os.path.join(HERE, "api", "mcg_main.cpp"), "-L", HERE, "-lmcg",
+          "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER])
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

@@ -1,0 +1,131 @@
+"""The C++ API mirror (montecarlocpp_amd/api/montecarlocpp.hpp) and the
+reference-compatible command line (api/mcg_main): the scene a reference deck
+builds through the mirror classes must equal the one the Python builders
+produce (same cells, faces, temperatures, pairs), and on the GPU the driver's
+printed solution must equal the CPU checker's for the same Philox key."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from montecarlocpp_amd import abi, build, materials, scenes
+
+DRIVER = build.DRIVER
+
+
+def write_materials(d):
+    grey = materials.grey(vel=3000.0, mfp=5e-8, nbins=12)
+    materials.write_reference_files(grey, d / "grey_disp.txt", d / "grey_relax2.txt")
+    si = materials.si_like(300.0, nbins=200)
+    materials.write_reference_files(si, d / "Si_disp.txt", d / "Si_relax2.txt")
+    return grey, si
+
+
+def run(args, env=None, check=True):
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True, text=True, env=e,
+                       timeout=300)
+    if check:
+        assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def spec_dict(dom):
+    d = dom.spec()
+    cells = []
+    for s in range(d.nsdom):
+        c = d.sdom[s]
+        cells.append({"cell": c.cell, "origin": list(c.origin), "mat": list(c.mat), "div": list(c.div),
+                      "grad_t": list(c.grad_t),
+                      "faces": [[c.face[f].kind, c.face[f].temp, c.face[f].specularity]
+                                for f in range(c.nface)]})
+    pairs = [[d.pair[k].sdom_a, d.pair[k].face_a, d.pair[k].sdom_b, d.pair[k].face_b,
+              *list(d.pair[k].transl)] for k in range(d.npair)]
+    return {"cells": cells, "pairs": pairs}
+
+
+def dump(tmp_path, args):
+    r = run([tmp_path, "grey", 300] + args, env={"MCG_DUMP_SCENE": "1"})
+    line = [l for l in r.stdout.splitlines() if l.startswith("{")][0]
+    return json.loads(line)
+
+
+@pytest.fixture()
+def deckdir(tmp_path):
+    write_materials(tmp_path)
+    return tmp_path
+
+
+@pytest.mark.parametrize("args,py", [
+    (["bulk", 2e-7, 8], lambda: scenes.bulk((2e-7, 2e-7, 2e-7), (8, 0, 0), 1e6 * 2e-7)),
+    (["film", 2e-7, 5e-8, 6], lambda: scenes.film((2e-7, 5e-8, 2e-7), (0, 6, 0), 1e6 * 2e-7)),
+    (["slab", 1e-7, 1e-7, 50, 1, -1], lambda: scenes.slab(1e-7, 50)),
+    (["bar", 1e-6, 1e-7, 100, 10, -10], lambda: scenes.bar(1e-6, 1e-7, 1e-7, 100)),
+])
+def test_mirror_builds_the_same_scene(deckdir, args, py):
+    got = dump(deckdir, args)
+    want = spec_dict(py())
+    assert len(got["cells"]) == len(want["cells"])
+    for g, w in zip(got["cells"], want["cells"]):
+        assert g["cell"] == w["cell"] and g["div"] == w["div"]
+        np.testing.assert_allclose(g["origin"], w["origin"], rtol=1e-12)
+        np.testing.assert_allclose(g["mat"], w["mat"], rtol=1e-12)
+        np.testing.assert_allclose(g["grad_t"], w["grad_t"], rtol=1e-12)
+        np.testing.assert_allclose(np.array(g["faces"]), np.array(w["faces"]), rtol=1e-12)
+    assert len(got["pairs"]) == len(want["pairs"])
+    for g, w in zip(got["pairs"], want["pairs"]):
+        assert g[:4] == w[:4]
+        np.testing.assert_allclose(g[4:], w[4:], rtol=1e-12)
+
+
+def test_driver_errors(deckdir):
+    r = run([deckdir, "grey", 300, "hex", 1e-7, 1e-7, "temp", 10, 10, 0, 1], check=False)
+    assert r.returncode == 1 and "Invalid domain" in r.stderr
+    r = run([deckdir, "custom", 300, "nope.txt", "nope2.txt"], check=False)
+    assert r.returncode == 1 and "Error opening dispersion file" in r.stderr
+
+
+def parse_solution(out):
+    lines = out.splitlines()
+    k = lines.index("Output")
+    rows = []
+    for l in lines[k + 1:]:
+        if not l.strip():
+            break
+        rows.append([float(t) for t in l.split()])
+    key = int(re.search(r"key: (\d+)", out).group(1))
+    return np.array(rows), key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("deck", [
+    ["silicon", 300, "film", 2e-7, 5e-8, 6, "multi", 20000, 40, 0, 1],
+    ["silicon", 300, "bar", 4e-7, 1e-7, 20, 10, -10, "multi", 20000, 1000000000, 0, 1],
+    ["grey", 300, "bulk", 2e-7, 8, "cumflux", 10000, 4, 9, 0, 1],
+])
+def test_driver_matches_checker(deckdir, deck):
+    from oracle import oracle
+    print("deck:", deck, flush=True)
+    r = run([deckdir] + deck, env={"MCG_SEED": "7", "MCG_PRECISION": "fp64"})
+    sol, key = parse_solution(r.stdout)
+    mat = (materials.read_reference_files(deckdir / "Si_disp.txt", deckdir / "Si_relax2.txt", 300.0)
+           if deck[0] == "silicon" else
+           materials.read_reference_files(deckdir / "grey_disp.txt", deckdir / "grey_relax2.txt", 300.0))
+    kind = deck[2]
+    if kind == "film":
+        dom = scenes.film((deck[3], deck[4], deck[3]), (0, deck[5], 0), 1e6 * deck[3])
+        tally, nemit, maxscat, size = abi.MULTI, deck[7], deck[8], 0
+    elif kind == "bar":
+        dom = scenes.bar(deck[3], deck[4], deck[4], deck[5], deck[6], deck[7])
+        tally, nemit, maxscat, size = abi.MULTI, deck[9], deck[10], 0
+    else:
+        dom = scenes.bulk((deck[3],) * 3, (deck[4], 0, 0), 1e6 * deck[3])
+        tally, nemit, size, maxscat = abi.CUMFLUX, deck[6], deck[7], deck[8]
+    f, _, _ = oracle.solve(mat, dom, scenes.problem(nemit, tally=tally, maxscat=maxscat, size=size,
+                                                      seed=key))
+    assert sol.shape == f.shape
+    np.testing.assert_allclose(sol, f, rtol=1e-8, atol=1e-9 * np.abs(f).max())
