@@ -223,6 +223,15 @@ int mcg_scene_create(const mcg_material_spec* mat, const mcg_domain_spec* dom,
                      mcg_scene** out);
 int mcg_scene_destroy(mcg_scene* scene);
 
+/* Launch shape of the scene's single-solve kernel (block size, bundle ids
+ * per wavefront grab = chunk >= 64, workgroups per CU); NULL or zeros
+ * restore the defaults.  Results do not depend on it. */
+typedef struct mcg_tune mcg_tune;
+int mcg_scene_tune(mcg_scene* scene, const mcg_tune* tune);
+
+/* mcg_problem_plan for a problem on an existing scene (no table rebuild). */
+int mcg_scene_plan(const mcg_scene* scene, const mcg_problem_spec* prob, mcg_problem_info* info);
+
 /* FieldProblem::solve (problem.cpp:370-445) for the whole bundle set:
  * out receives rows x cols doubles, column-major, equal to the reference's
  * return value power_ * postProc(data) / cellVol.  counts may be NULL. */
@@ -244,12 +253,13 @@ int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, do
 /* --------------------------------------------------------------- batching
  * Many independent solves in one persistent launch.  Each solve has its
  * own scene, problem and output; results equal those of mcg_solve. */
-typedef struct mcg_tune {
-    int32_t block;             /* threads per block: 64, 128, 256, 512 (0 = auto) */
+struct mcg_tune {
+    int32_t block;             /* threads per block: 64 ... 1024 (0 = auto)    */
     int32_t per_thread;        /* phonons carried per thread: 1, 2, 4, 8        */
     int32_t chunk;             /* bundles claimed per wavefront grab            */
     int32_t blocks_per_cu;     /* occupancy target (0 = auto)                   */
-} mcg_tune;
+    int32_t lds_tables;        /* sampling tables in LDS: 1 yes, -1 no, 0 auto  */
+};
 
 int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
                     double* const* outs, const int64_t* out_lens,

@@ -91,7 +91,7 @@ class Stats(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("per_thread", C.c_int32), ("chunk", C.c_int32),
-                ("blocks_per_cu", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("lds_tables", C.c_int32)]
 
 
 def dptr(a):

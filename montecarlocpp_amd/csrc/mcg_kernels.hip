@@ -62,6 +62,7 @@ struct KParams {
     uint32_t* c_nstep;
     uint32_t* c_fate;
     int32_t lds_tally;
+    int32_t chunk;         // bundle ids a wavefront claims at a time
     int64_t trace_id;      // bundle whose steps are recorded, -1 for none
     double* trace;         // MCG_TRACE_WIDTH doubles per step
     int64_t trace_cap;     // records available
@@ -343,14 +344,25 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// A workgroup's view of one solve's scene: LDS copies when staged.
+template <typename F>
+struct View {
+    const DevPlane<F>* planes;
+    const DevSdom<F>* sdoms;
+    const uint64_t* scat;
+    const F* vel;
+    const F* mfp;
+    double* direct;
+    double* diff;
+};
+
+// Stage one solve's tables, planes and subdomains in LDS and clear the LDS
+// tally; the caller synchronises.  Layout: [scat alias | vel | mfp] (when
+// kLdsTables), planes, subdomains, tally direct, tally diff.
 template <typename F, bool kLdsTables, bool kLdsTally>
-__global__ void __launch_bounds__(256) trajectory_kernel(KParams<F> K) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    // ---- stage tables in LDS
+__device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int nthreads) {
     const int ntab = kLdsTables ? K.nmodes : 0;
-    const uint64_t* s_scat = kLdsTables ? reinterpret_cast<const uint64_t*>(lds) : K.scat_alias;
-    const F* s_vel = kLdsTables ? reinterpret_cast<const F*>(lds + 8 * size_t(ntab)) : K.mode_vel;
-    const F* s_mfp = kLdsTables ? s_vel + ntab : K.mode_mfp;
+    View<F> v;
     size_t off = (size_t(ntab) * (8 + 2 * sizeof(F)) + 15) & ~size_t(15);
     DevPlane<F>* s_planes = reinterpret_cast<DevPlane<F>*>(lds + off);
     off = (off + sizeof(DevPlane<F>) * K.nplanes + 15) & ~size_t(15);
@@ -358,254 +370,377 @@ __global__ void __launch_bounds__(256) trajectory_kernel(KParams<F> K) {
     off = (off + sizeof(DevSdom<F>) * K.nsdom + 15) & ~size_t(15);
     double* s_direct = reinterpret_cast<double*>(lds + off);
     double* s_diff = s_direct + size_t(K.rows) * K.cols;
-    const int tid = threadIdx.x;
     if (kLdsTables) {
         uint64_t* w_scat = reinterpret_cast<uint64_t*>(lds);
         F* w_vel = reinterpret_cast<F*>(lds + 8 * size_t(ntab));
         F* w_mfp = w_vel + ntab;
-        for (int i = tid; i < ntab; i += blockDim.x) {
+        for (int i = tid; i < ntab; i += nthreads) {
             w_scat[i] = K.scat_alias[i];
             w_vel[i] = K.mode_vel[i];
             w_mfp[i] = K.mode_mfp[i];
         }
+        v.scat = w_scat;
+        v.vel = w_vel;
+        v.mfp = w_mfp;
+    } else {
+        v.scat = K.scat_alias;
+        v.vel = K.mode_vel;
+        v.mfp = K.mode_mfp;
     }
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(K.planes);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(s_planes);
-        const int nw = int(sizeof(DevPlane<F>) / 4) * K.nplanes;
-        for (int i = tid; i < nw; i += blockDim.x) dst[i] = src[i];
-        src = reinterpret_cast<const uint32_t*>(K.sdoms);
-        dst = reinterpret_cast<uint32_t*>(s_sdoms);
-        const int ns = int(sizeof(DevSdom<F>) / 4) * K.nsdom;
-        for (int i = tid; i < ns; i += blockDim.x) dst[i] = src[i];
-    }
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(K.planes);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s_planes);
+    const int np = int(sizeof(DevPlane<F>) / 4) * K.nplanes;
+    for (int i = tid; i < np; i += nthreads) dst[i] = src[i];
+    src = reinterpret_cast<const uint32_t*>(K.sdoms);
+    dst = reinterpret_cast<uint32_t*>(s_sdoms);
+    const int ns = int(sizeof(DevSdom<F>) / 4) * K.nsdom;
+    for (int i = tid; i < ns; i += nthreads) dst[i] = src[i];
     if (kLdsTally) {
         const int n = K.rows * K.cols + K.rows * (K.cols + 1);
-        for (int i = tid; i < n; i += blockDim.x) s_direct[i] = 0.0;
+        for (int i = tid; i < n; i += nthreads) s_direct[i] = 0.0;
+        v.direct = s_direct;
+        v.diff = s_diff;
+    } else {
+        v.direct = K.direct;
+        v.diff = K.diff;
     }
-    const Sink sink = kLdsTally ? Sink{s_direct, s_diff, K.cols} : Sink{K.direct, K.diff, K.cols};
+    v.planes = s_planes;
+    v.sdoms = s_sdoms;
+    return v;
+}
+
+// Add a workgroup's LDS tally into the solve's global tally (after a barrier).
+__device__ void flush_tally(const double* s_direct, const double* s_diff, double* g_direct,
+                            double* g_diff, int rows, int cols, int tid, int nthreads) {
+    const int nd = rows * cols;
+    for (int i = tid; i < nd; i += nthreads) {
+        const double v = s_direct[i];
+        if (v != 0.0) unsafeAtomicAdd(g_direct + i, v);
+    }
+    const int nf = rows * (cols + 1);
+    for (int i = tid; i < nf; i += nthreads) {
+        const double v = s_diff[i];
+        if (v != 0.0) unsafeAtomicAdd(g_diff + i, v);
+    }
+}
+
+struct LaneStats {
+    unsigned long long steps = 0, esc = 0, absorbed = 0, trunc = 0, bundles = 0;
+    __device__ void add(uint32_t fate, uint32_t nstep) {
+        steps += nstep;
+        bundles++;
+        esc += fate == MCG_ESCAPED;
+        absorbed += fate == MCG_ABSORBED;
+        trunc += fate == MCG_MAXSCAT || fate == MCG_MAXLOOP;
+    }
+    // wave-reduce and add to the solve's counters (all lanes call)
+    __device__ void publish(unsigned long long* counters, int lane) {
+        const unsigned long long a = wave_sum(steps), b = wave_sum(esc), c = wave_sum(absorbed),
+                                 d = wave_sum(trunc), e = wave_sum(bundles);
+        if (lane == 0 && e != 0) {
+            atomicAdd(counters + C_STEPS, a);
+            atomicAdd(counters + C_ESC, b);
+            atomicAdd(counters + C_ABS, c);
+            atomicAdd(counters + C_TRUNC, d);
+            atomicAdd(counters + C_BUNDLES, e);
+        }
+        steps = esc = absorbed = trunc = bundles = 0;
+    }
+};
+
+// One scatter-step of the bundle in L (problem.cpp:401-435): advect, tally,
+// boundary or intrinsic scatter, termination test.  Returns the bundle's fate
+// when it ends, 0xFFFFFFFF while it lives.
+template <typename F>
+__device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View<F>& V,
+                                                 const Sink& sink, Lane<F>& L) {
+    const DevSdom<F>& S = V.sdoms[L.sdom];
+    const F vel = V.vel[L.mode];
+    // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
+    F dmin = L.scat_next;
+    int hit = -1;
+    for (int b = S.pbeg; b < S.pend; ++b) {
+        const DevPlane<F>& P = V.planes[b];
+        const F nd = dev::dot3(P.n, L.dir);
+        if (nd >= F(0)) continue;
+        const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
+        if (t < dmin) {
+            dmin = t;
+            hit = b;
+        }
+    }
+    if (S.pore >= 0) {
+        const DevPore<F>& Q = K.pores[S.pore];
+        const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
+        const F bq = dx * L.dir.x + dy * L.dir.y;
+        if (bq < F(0)) {
+            const F aq = L.dir.x * L.dir.x + L.dir.y * L.dir.y;
+            const F cq = dx * dx + dy * dy - Q.r2;
+            const F disc = bq * bq - aq * cq;
+            if (disc >= F(0)) {
+                const F t = cq / (dev::Real<F>::sqrt_(disc) - bq);
+                if (t < dmin) {
+                    dmin = t;
+                    hit = -3;
+                }
+            }
+        }
+    }
+    L.scat_next -= dmin;
+    if (L.scat_next < dev::Real<F>::kMin) L.scat_next = F(0);
+    const dev::V3<F> pre = L.pos;
+    L.pos = dev::V3<F>{pre.x + dmin * L.dir.x, pre.y + dmin * L.dir.y, pre.z + dmin * L.dir.z};
+    L.nstep++;
+    const dev::V3<F> apos = L.pos;
+    uint32_t fate = 0xFFFFFFFFu;
+    if (dmin < -S.eps || !inside(V.planes, S, K.pores, L.pos)) {
+        fate = MCG_ESCAPED;
+    } else {
+        // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637)
+        const double sg = double(L.sign);
+        const double dt = sg * double(dmin / vel);
+        const double dx = sg * double(L.pos.x - pre.x);
+        const double dy = sg * double(L.pos.y - pre.y);
+        const double dz = sg * double(L.pos.z - pre.z);
+        const int tk = K.tally;
+        const bool with_t = tk == MCG_TEMP || tk == MCG_MULTI || tk == MCG_CUMTEMP;
+        const double amt[kMaxRows] = {with_t ? dt : dx, with_t ? dx : dy, with_t ? dy : dz, dz};
+        const int nv = tk == MCG_MULTI ? 4 : (with_t ? 1 : 3);
+        int r0 = 0;
+        if (tk >= MCG_CUMTEMP) {
+            const int bin = int((int64_t(L.nscat) + K.step_cum - 1) / K.step_cum);
+            r0 = tk == MCG_CUMTEMP ? bin : 3 * bin;
+        }
+        accumulate<F>(S, sink, pre, L.pos, r0, nv, amt);
+
+        // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
+        const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
+        if (hit == -1) {
+            const uint4 b0 = ev.block(0), b1 = ev.block(1);
+            L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
+            L.dir = dev::normalized(dev::draw_iso<F>(b0));
+            L.nscat++;
+            L.scat_next = draw_path<F>(ev, b1, V.mfp[L.mode]);
+        } else if (hit >= 0) {
+            L.nbdry++;
+            const DevPlane<F>& P = V.planes[hit];
+            int kind = P.kind;
+            if (kind == MCG_PARTIAL) kind = ev.block(1).w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+            switch (kind) {
+                case MCG_SPEC:   // boundary.cpp:283-287
+                    L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
+                    break;
+                case MCG_DIFF:   // boundary.cpp:308-312
+                    L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(ev.block(0), false)));
+                    L.nscat++;
+                    break;
+                case MCG_ISOT:   // boundary.cpp:456-460
+                    fate = MCG_ABSORBED;
+                    break;
+                case MCG_PERI: { // boundary.cpp:517-522
+                    const dev::V3<F> q = dev::matvec(P.prot, L.pos);
+                    L.pos = dev::V3<F>{q.x + P.pt[0], q.y + P.pt[1], q.z + P.pt[2]};
+                    L.dir = dev::normalized(dev::matvec(P.prot, L.dir));
+                    L.sdom = P.target;
+                    break;
+                }
+                default: {       // MCG_INTER, boundary.cpp:349-359
+                    int target = -1;
+                    if (P.icnt == 1) {
+                        target = K.inter_list[P.ibeg];
+                    } else {
+                        for (int q = 0; q < P.icnt; ++q) {
+                            const int cand = K.inter_list[P.ibeg + q];
+                            if (inside(V.planes, V.sdoms[cand], K.pores, L.pos)) {
+                                target = cand;
+                                break;
+                            }
+                        }
+                    }
+                    if (target < 0) fate = MCG_ESCAPED;
+                    else L.sdom = target;
+                    break;
+                }
+            }
+        } else {
+            L.nbdry++;
+            pore_scatter<F>(K.pores[S.pore], ev, L);
+        }
+        if (fate == 0xFFFFFFFFu) {
+            if (int64_t(L.nscat) >= K.maxscat) fate = MCG_MAXSCAT;
+            else if (L.nstep >= K.maxloop) fate = MCG_MAXLOOP;
+        }
+    }
+    if (L.id == K.trace_id && int64_t(L.nstep) <= K.trace_cap) {
+        double* r = K.trace + (int64_t(L.nstep) - 1) * MCG_TRACE_WIDTH;
+        r[0] = double(L.nstep);
+        r[1] = double(hit);
+        r[2] = double(dmin);
+        r[3] = double(apos.x);
+        r[4] = double(apos.y);
+        r[5] = double(apos.z);
+        r[6] = double(L.dir.x);
+        r[7] = double(L.dir.y);
+        r[8] = double(L.dir.z);
+        r[9] = double(L.scat_next);
+        r[10] = double(L.mode);
+        r[11] = double(L.nscat);
+        r[12] = double(L.sdom);
+        r[13] = fate == 0xFFFFFFFFu ? -1.0 : double(fate);
+        r[14] = double(L.pos.x);
+        r[15] = double(L.pos.y);
+    }
+    if (fate != 0xFFFFFFFFu && K.c_nscat) {
+        const int64_t k = L.id - K.first;
+        K.c_nscat[k] = L.nscat;
+        K.c_nbdry[k] = L.nbdry;
+        K.c_nstep[k] = L.nstep;
+        K.c_fate[k] = fate;
+    }
+    return fate;
+}
+
+// Wave-level id pool: lanes whose bundle ended take the next ids of the pool,
+// refilled `chunk` ids at a time from *counter by one lane.  Returns the id
+// offset for lanes with `need` (all lanes call, the pool stays wave-uniform).
+template <typename Counter>
+__device__ __forceinline__ int64_t claim(bool need, int lane, uint64_t lt_mask, int64_t& base,
+                                         int64_t& lim, Counter* counter, int chunk) {
+    const uint64_t mask = __ballot(need);
+    if (!mask) return 0;
+    const int cnt = __popcll(mask);
+    const int rank = __popcll(mask & lt_mask);
+    const int64_t avail = lim - base;
+    int64_t mine;
+    if (avail >= cnt) {
+        mine = base + rank;
+        base += cnt;
+    } else {
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long fresh = 0;
+        if (lane == leader) fresh = (unsigned long long)atomicAdd(counter, Counter(chunk));
+        fresh = __shfl(fresh, leader);
+        mine = rank < avail ? base + rank : int64_t(fresh) + (rank - avail);
+        base = int64_t(fresh) + (cnt - avail);
+        lim = int64_t(fresh) + chunk;
+    }
+    return mine;
+}
+
+// Single solve: persistent grid, every lane refilled from the solve's global
+// bundle counter until the range [first, first + count) is exhausted.
+template <typename F, bool kLdsTables, bool kLdsTally>
+__global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tid = threadIdx.x;
+    const View<F> V = stage<F, kLdsTables, kLdsTally>(K, lds, tid, blockDim.x);
+    const Sink sink{V.direct, V.diff, K.cols};
     __syncthreads();
 
     const int lane = tid & (kWave - 1);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-    int64_t pool_base = 0, pool_lim = 0;   // wave-uniform id pool
+    int64_t pool_base = 0, pool_lim = 0;
     bool active = false, finished = false;
     Lane<F> L;
-    unsigned long long n_steps = 0, n_esc = 0, n_abs = 0, n_trunc = 0, n_bundles = 0;
-
+    LaneStats st;
     for (;;) {
-        // ---- refill lanes whose bundle ended
         const bool need = !active && !finished;
-        const uint64_t mask = __ballot(need);
-        if (mask) {
-            const int cnt = __popcll(mask);
-            const int rank = __popcll(mask & lt_mask);
-            const int64_t avail = pool_lim - pool_base;
-            int64_t mine;
-            if (avail >= cnt) {
-                mine = pool_base + rank;
-                pool_base += cnt;
+        const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.counters + C_NEXT, K.chunk);
+        if (need) {
+            if (mine < K.count) {
+                emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
+                active = true;
             } else {
-                const int leader = __ffsll((long long)mask) - 1;
-                unsigned long long fresh = 0;
-                if (lane == leader) fresh = atomicAdd(K.counters + C_NEXT, (unsigned long long)kChunk);
-                fresh = __shfl(fresh, leader);
-                mine = rank < avail ? pool_base + rank : int64_t(fresh) + (rank - avail);
-                pool_base = int64_t(fresh) + (cnt - avail);
-                pool_lim = int64_t(fresh) + kChunk;
+                finished = true;
             }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (!active) continue;
+        const uint32_t fate = scatter_step<F>(K, V, sink, L);
+        if (fate != 0xFFFFFFFFu) {
+            st.add(fate, L.nstep);
+            active = false;
+        }
+    }
+    st.publish(K.counters, lane);
+    if (kLdsTally) {
+        __syncthreads();
+        flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, tid, blockDim.x);
+    }
+}
+
+// One unit of batched work: bundles [begin, begin + count) of solve `solve`.
+struct WorkItem {
+    int32_t solve;
+    int32_t count;
+    int64_t begin;
+};
+
+// Many solves in one persistent launch.  A workgroup claims work items in
+// order; when the item's solve differs from the one staged in its LDS it
+// flushes the staged tally and stages the new solve.  Within an item the
+// waves share the item's bundles through an LDS counter.
+template <typename F, bool kLdsTables, bool kLdsTally>
+__global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restrict__ solves,
+                                                    const WorkItem* __restrict__ items, int nitems,
+                                                    unsigned int* item_counter, int chunk) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int s_item;
+    __shared__ unsigned int s_next;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    int cur = -1;
+    View<F> V{};
+    LaneStats st;
+    for (;;) {
+        if (tid == 0) {
+            s_item = int(atomicAdd(item_counter, 1u));
+            s_next = 0u;
+        }
+        __syncthreads();
+        const int item = s_item;
+        if (item >= nitems) break;
+        const WorkItem w = items[item];
+        if (w.solve != cur) {
+            if (cur >= 0 && kLdsTally)
+                flush_tally(V.direct, V.diff, solves[cur].direct, solves[cur].diff, solves[cur].rows,
+                            solves[cur].cols, tid, blockDim.x);
+            __syncthreads();
+            V = stage<F, kLdsTables, kLdsTally>(solves[w.solve], lds, tid, blockDim.x);
+            cur = w.solve;
+            __syncthreads();
+        }
+        const KParams<F>& K = solves[cur];
+        const Sink sink{V.direct, V.diff, K.cols};
+        int64_t pool_base = 0, pool_lim = 0;
+        bool active = false, finished = false;
+        Lane<F> L;
+        for (;;) {
+            const bool need = !active && !finished;
+            const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, &s_next, chunk);
             if (need) {
-                if (mine < K.count) {
-                    emit<F>(K, K.flux_alias, s_mfp, K.first + mine, L);
+                if (mine < w.count) {
+                    emit<F>(K, K.flux_alias, V.mfp, w.begin + mine, L);
                     active = true;
                 } else {
                     finished = true;
                 }
             }
-        }
-        if (__ballot(active) == 0ull) break;
-        if (!active) continue;
-
-        // ---- one scatter-step (problem.cpp:401-435)
-        const DevSdom<F>& S = s_sdoms[L.sdom];
-        const F vel = s_vel[L.mode];
-        // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
-        F dmin = L.scat_next;
-        int hit = -1;
-        for (int b = S.pbeg; b < S.pend; ++b) {
-            const DevPlane<F>& P = s_planes[b];
-            const F nd = dev::dot3(P.n, L.dir);
-            if (nd >= F(0)) continue;
-            const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
-            if (t < dmin) {
-                dmin = t;
-                hit = b;
+            if (__ballot(active) == 0ull) break;
+            if (!active) continue;
+            const uint32_t fate = scatter_step<F>(K, V, sink, L);
+            if (fate != 0xFFFFFFFFu) {
+                st.add(fate, L.nstep);
+                active = false;
             }
         }
-        if (S.pore >= 0) {
-            const DevPore<F>& Q = K.pores[S.pore];
-            const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
-            const F bq = dx * L.dir.x + dy * L.dir.y;
-            if (bq < F(0)) {
-                const F aq = L.dir.x * L.dir.x + L.dir.y * L.dir.y;
-                const F cq = dx * dx + dy * dy - Q.r2;
-                const F disc = bq * bq - aq * cq;
-                if (disc >= F(0)) {
-                    const F t = cq / (dev::Real<F>::sqrt_(disc) - bq);
-                    if (t < dmin) {
-                        dmin = t;
-                        hit = -3;
-                    }
-                }
-            }
-        }
-        L.scat_next -= dmin;
-        if (L.scat_next < dev::Real<F>::kMin) L.scat_next = F(0);
-        const dev::V3<F> pre = L.pos;
-        L.pos = dev::V3<F>{pre.x + dmin * L.dir.x, pre.y + dmin * L.dir.y, pre.z + dmin * L.dir.z};
-        L.nstep++;
-        const dev::V3<F> apos = L.pos;
-        uint32_t fate = 0xFFFFFFFFu;
-        if (dmin < -S.eps || !inside(s_planes, S, K.pores, L.pos)) {
-            fate = MCG_ESCAPED;
-        } else {
-            // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637)
-            const double sg = double(L.sign);
-            const double dt = sg * double(dmin / vel);
-            const double dx = sg * double(L.pos.x - pre.x);
-            const double dy = sg * double(L.pos.y - pre.y);
-            const double dz = sg * double(L.pos.z - pre.z);
-            const int tk = K.tally;
-            const bool with_t = tk == MCG_TEMP || tk == MCG_MULTI || tk == MCG_CUMTEMP;
-            const double amt[kMaxRows] = {with_t ? dt : dx, with_t ? dx : dy, with_t ? dy : dz, dz};
-            const int nv = tk == MCG_MULTI ? 4 : (with_t ? 1 : 3);
-            int r0 = 0;
-            if (tk >= MCG_CUMTEMP) {
-                const int bin = int((int64_t(L.nscat) + K.step_cum - 1) / K.step_cum);
-                r0 = tk == MCG_CUMTEMP ? bin : 3 * bin;
-            }
-            accumulate<F>(S, sink, pre, L.pos, r0, nv, amt);
-
-            // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
-            const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
-            if (hit == -1) {
-                const uint4 b0 = ev.block(0), b1 = ev.block(1);
-                L.mode = dev::alias_draw(s_scat, uint32_t(K.nmodes), b1.z);
-                L.dir = dev::normalized(dev::draw_iso<F>(b0));
-                L.nscat++;
-                L.scat_next = draw_path<F>(ev, b1, s_mfp[L.mode]);
-            } else if (hit >= 0) {
-                L.nbdry++;
-                const DevPlane<F>& P = s_planes[hit];
-                int kind = P.kind;
-                if (kind == MCG_PARTIAL) kind = ev.block(1).w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
-                switch (kind) {
-                    case MCG_SPEC:   // boundary.cpp:283-287
-                        L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
-                        break;
-                    case MCG_DIFF:   // boundary.cpp:308-312
-                        L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(ev.block(0), false)));
-                        L.nscat++;
-                        break;
-                    case MCG_ISOT:   // boundary.cpp:456-460
-                        fate = MCG_ABSORBED;
-                        break;
-                    case MCG_PERI: { // boundary.cpp:517-522
-                        const dev::V3<F> q = dev::matvec(P.prot, L.pos);
-                        L.pos = dev::V3<F>{q.x + P.pt[0], q.y + P.pt[1], q.z + P.pt[2]};
-                        L.dir = dev::normalized(dev::matvec(P.prot, L.dir));
-                        L.sdom = P.target;
-                        break;
-                    }
-                    default: {       // MCG_INTER, boundary.cpp:349-359
-                        int target = -1;
-                        if (P.icnt == 1) {
-                            target = K.inter_list[P.ibeg];
-                        } else {
-                            for (int q = 0; q < P.icnt; ++q) {
-                                const int cand = K.inter_list[P.ibeg + q];
-                                if (inside(s_planes, s_sdoms[cand], K.pores, L.pos)) {
-                                    target = cand;
-                                    break;
-                                }
-                            }
-                        }
-                        if (target < 0) fate = MCG_ESCAPED;
-                        else L.sdom = target;
-                        break;
-                    }
-                }
-            } else {
-                L.nbdry++;
-                pore_scatter<F>(K.pores[S.pore], ev, L);
-            }
-            if (fate == 0xFFFFFFFFu) {
-                if (int64_t(L.nscat) >= K.maxscat) fate = MCG_MAXSCAT;
-                else if (L.nstep >= K.maxloop) fate = MCG_MAXLOOP;
-            }
-        }
-        if (L.id == K.trace_id && int64_t(L.nstep) <= K.trace_cap) {
-            double* r = K.trace + (int64_t(L.nstep) - 1) * MCG_TRACE_WIDTH;
-            r[0] = double(L.nstep);
-            r[1] = double(hit);
-            r[2] = double(dmin);
-            r[3] = double(apos.x);
-            r[4] = double(apos.y);
-            r[5] = double(apos.z);
-            r[6] = double(L.dir.x);
-            r[7] = double(L.dir.y);
-            r[8] = double(L.dir.z);
-            r[9] = double(L.scat_next);
-            r[10] = double(L.mode);
-            r[11] = double(L.nscat);
-            r[12] = double(L.sdom);
-            r[13] = fate == 0xFFFFFFFFu ? -1.0 : double(fate);
-            r[14] = double(L.pos.x);
-            r[15] = double(L.pos.y);
-        }
-        if (fate != 0xFFFFFFFFu) {
-            n_steps += L.nstep;
-            n_bundles++;
-            n_esc += fate == MCG_ESCAPED;
-            n_abs += fate == MCG_ABSORBED;
-            n_trunc += fate == MCG_MAXSCAT || fate == MCG_MAXLOOP;
-            if (K.c_nscat) {
-                const int64_t k = L.id - K.first;
-                K.c_nscat[k] = L.nscat;
-                K.c_nbdry[k] = L.nbdry;
-                K.c_nstep[k] = L.nstep;
-                K.c_fate[k] = fate;
-            }
-            active = false;
-        }
-    }
-
-    // ---- counters and tally flush
-    n_steps = wave_sum(n_steps);
-    n_esc = wave_sum(n_esc);
-    n_abs = wave_sum(n_abs);
-    n_trunc = wave_sum(n_trunc);
-    n_bundles = wave_sum(n_bundles);
-    if (lane == 0) {
-        atomicAdd(K.counters + C_STEPS, n_steps);
-        atomicAdd(K.counters + C_ESC, n_esc);
-        atomicAdd(K.counters + C_ABS, n_abs);
-        atomicAdd(K.counters + C_TRUNC, n_trunc);
-        atomicAdd(K.counters + C_BUNDLES, n_bundles);
-    }
-    if (kLdsTally) {
+        st.publish(K.counters, lane);
         __syncthreads();
-        const int nd = K.rows * K.cols;
-        for (int i = tid; i < nd; i += blockDim.x) {
-            const double v = s_direct[i];
-            if (v != 0.0) unsafeAtomicAdd(K.direct + i, v);
-        }
-        const int nf = K.rows * (K.cols + 1);
-        for (int i = tid; i < nf; i += blockDim.x) {
-            const double v = s_diff[i];
-            if (v != 0.0) unsafeAtomicAdd(K.diff + i, v);
-        }
     }
+    if (cur >= 0 && kLdsTally)
+        flush_tally(V.direct, V.diff, solves[cur].direct, solves[cur].diff, solves[cur].rows,
+                    solves[cur].cols, tid, blockDim.x);
 }
 
 // ------------------------------------------------------------------ host
@@ -725,6 +860,7 @@ struct mcg_scene {
     int device = 0;
     int num_cu = 0;
     size_t lds_limit = 0;
+    mcg_tune tune{0, 0, 0, 0, 0};   // launch shape for this scene's solves (0 = default)
 };
 
 namespace mcg {
@@ -740,11 +876,48 @@ size_t lds_bytes(const HostScene& h, int rows, bool tables, bool tally) {
 }
 
 template <typename F>
+DevScene<F>& dev_scene(mcg_scene* sc) {
+    return sizeof(F) == 8 ? reinterpret_cast<DevScene<F>&>(sc->d64)
+                          : reinterpret_cast<DevScene<F>&>(sc->d32);
+}
+
+// Scene and problem fields of a solve's kernel parameters (buffers aside).
+template <typename F>
+void fill_params(mcg_scene* sc, const HostProblem& p, KParams<F>& K) {
+    const HostScene& h = sc->host;
+    DevScene<F>& d = dev_scene<F>(sc);
+    K.planes = d.planes.p;
+    K.sdoms = d.sdoms.p;
+    K.emitters = d.emitters.p;
+    K.pores = d.pores.p;
+    K.inter_list = sc->inter_list.p;
+    K.mode_vel = d.mode_vel.p;
+    K.mode_mfp = d.mode_mfp.p;
+    K.scat_alias = sc->scat_alias.p;
+    K.flux_alias = sc->flux_alias.p;
+    K.nplanes = int32_t(h.planes.size());
+    K.nsdom = int32_t(h.sdoms.size());
+    K.nemitter = int32_t(h.emitters.size());
+    K.npore = int32_t(h.pores.size());
+    K.nmodes = int32_t(h.mode_cell.size());
+    K.tally = p.tally;
+    K.rows = int32_t(p.rows);
+    K.cols = int32_t(h.cols);
+    K.step_cum = p.step > 0 ? p.step : 1;
+    K.maxscat = p.maxscat;
+    K.maxloop = uint32_t(std::min<int64_t>(p.maxloop, 0xFFFFFFFEll));
+    K.k0 = uint32_t(p.seed);
+    K.k1 = uint32_t(p.seed >> 32);
+    K.first = p.first;
+    K.count = p.count;
+    K.chunk = sc->tune.chunk > 0 ? sc->tune.chunk : kChunk;
+    K.trace_id = -1;
+}
+
+template <typename F>
 int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* counts, mcg_stats* stats,
               int64_t trace_id = -1, double* trace = nullptr, int64_t trace_cap = 0) {
     const HostScene& h = sc->host;
-    DevScene<F>& d = sizeof(F) == 8 ? reinterpret_cast<DevScene<F>&>(sc->d64)
-                                    : reinterpret_cast<DevScene<F>&>(sc->d32);
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipSetDevice(sc->device));
     const int rows = int(p.rows);
@@ -767,31 +940,8 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     HIP_TRY(hipMemset(ctr.p, 0, ctr.n * sizeof(unsigned long long)));
 
     KParams<F> K{};
-    K.planes = d.planes.p;
-    K.sdoms = d.sdoms.p;
-    K.emitters = d.emitters.p;
-    K.pores = d.pores.p;
+    fill_params<F>(sc, p, K);
     K.emit_cdf = cdf.p;
-    K.inter_list = sc->inter_list.p;
-    K.mode_vel = d.mode_vel.p;
-    K.mode_mfp = d.mode_mfp.p;
-    K.scat_alias = sc->scat_alias.p;
-    K.flux_alias = sc->flux_alias.p;
-    K.nplanes = int32_t(h.planes.size());
-    K.nsdom = int32_t(h.sdoms.size());
-    K.nemitter = int32_t(h.emitters.size());
-    K.npore = int32_t(h.pores.size());
-    K.nmodes = int32_t(h.mode_cell.size());
-    K.tally = p.tally;
-    K.rows = rows;
-    K.cols = cols;
-    K.step_cum = p.step > 0 ? p.step : 1;
-    K.maxscat = p.maxscat;
-    K.maxloop = uint32_t(std::min<int64_t>(p.maxloop, 0xFFFFFFFEll));
-    K.k0 = uint32_t(p.seed);
-    K.k1 = uint32_t(p.seed >> 32);
-    K.first = p.first;
-    K.count = p.count;
     K.direct = direct.p;
     K.diff = diff.p;
     K.counters = ctr.p;
@@ -802,7 +952,6 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
         K.c_fate = cnt.p + 3 * p.count;
     }
     DevBuf<double> tbuf;
-    K.trace_id = -1;
     if (trace && trace_cap > 0) {
         if ((rc = tbuf.alloc(size_t(trace_cap) * MCG_TRACE_WIDTH))) return rc;
         HIP_TRY(hipMemset(tbuf.p, 0, tbuf.n * sizeof(double)));
@@ -811,7 +960,7 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
         K.trace_cap = trace_cap;
     }
     // LDS plan: tables first, then the tally if it is small enough
-    const bool tables = lds_bytes<F>(h, rows, true, false) <= sc->lds_limit;
+    const bool tables = sc->tune.lds_tables >= 0 && lds_bytes<F>(h, rows, true, false) <= sc->lds_limit;
     const size_t base = lds_bytes<F>(h, rows, tables, false);
     const size_t with_tally = lds_bytes<F>(h, rows, tables, true);
     const bool tally = with_tally - base <= kLdsTallyCap && with_tally <= sc->lds_limit;
@@ -823,9 +972,10 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
                             : (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, false, true>)
                                      : reinterpret_cast<const void*>(trajectory_kernel<F, false, false>));
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    const int block = 256;
+    const int block = sc->tune.block > 0 ? sc->tune.block : 512;
     int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
-    blocks_per_cu = std::min(blocks_per_cu, 4);
+    blocks_per_cu = std::min(blocks_per_cu, std::max(1, 2048 / block));
+    if (sc->tune.blocks_per_cu > 0) blocks_per_cu = std::min(blocks_per_cu, int(sc->tune.blocks_per_cu));
     const int64_t needed = (p.count + block - 1) / block;
     const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
     hipEvent_t e0, e1;
@@ -873,6 +1023,157 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     }
     if (int64_t(hc[C_BUNDLES]) != p.count)
         return set_error(MCG_EDEVICE, "kernel finished with bundles unaccounted for");
+    return MCG_OK;
+}
+
+// Launch plan of a batch: per-solve tally offsets, work items, LDS layout.
+struct BatchPlan {
+    std::vector<size_t> off_direct, off_diff;
+    size_t n_direct = 0, n_diff = 0;
+    std::vector<WorkItem> items;
+    size_t lds = 0;
+    bool tables = true, tally = true;
+    int block = 256, grid = 0, chunk = 0, item = 0;
+};
+
+template <typename F>
+int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_tune* tune,
+               BatchPlan& B) {
+    const mcg_scene* s0 = scenes[0];
+    B.off_direct.resize(n);
+    B.off_diff.resize(n);
+    B.n_direct = B.n_diff = 0;
+    size_t lds_tables = 0, lds_plain = 0, lds_tally = 0, tally_bytes = 0;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const HostScene& h = scenes[i]->host;
+        if (scenes[i]->device != s0->device) return set_error(MCG_EINVAL, "batch scenes on different devices");
+        B.off_direct[i] = B.n_direct;
+        B.off_diff[i] = B.n_diff;
+        B.n_direct += size_t(P[i].rows) * h.cols;
+        B.n_diff += size_t(P[i].rows) * (h.cols + 1);
+        lds_tables = std::max(lds_tables, lds_bytes<F>(h, int(P[i].rows), true, false));
+        lds_plain = std::max(lds_plain, lds_bytes<F>(h, int(P[i].rows), false, false));
+        lds_tally = std::max(lds_tally, lds_bytes<F>(h, int(P[i].rows), true, true));
+        tally_bytes = std::max(tally_bytes, lds_bytes<F>(h, int(P[i].rows), false, true) -
+                                                lds_bytes<F>(h, int(P[i].rows), false, false));
+        total += P[i].count;
+    }
+    B.block = (tune && tune->block) ? tune->block : 512;
+    if (B.block != 64 && B.block != 128 && B.block != 256 && B.block != 512 && B.block != 1024)
+        return set_error(MCG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+    B.tables = !(tune && tune->lds_tables < 0) && lds_tables <= s0->lds_limit;
+    const size_t base = B.tables ? lds_tables : lds_plain;
+    B.tally = tally_bytes <= kLdsTallyCap && base + tally_bytes <= s0->lds_limit;
+    B.lds = base + (B.tally ? tally_bytes : 0);
+    if (B.lds > s0->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
+    // occupancy: LDS per CU (the per-block limit is the CU's LDS on gfx950)
+    // and 16 waves per CU for this kernel's register use
+    int per_cu = std::max<int>(1, int(s0->lds_limit / std::max<size_t>(B.lds, 1)));
+    per_cu = std::min(per_cu, std::max(1, 2048 / B.block));
+    if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, int(tune->blocks_per_cu));
+    B.grid = s0->num_cu * per_cu;
+    B.chunk = 64;
+    // work items: each solve's range cut into items of about total/(8 grid)
+    int64_t item = (tune && tune->chunk > 0) ? tune->chunk
+                                             : std::max<int64_t>(2048, total / (int64_t(B.grid) * 8));
+    item = std::min<int64_t>(item, 1 << 30);
+    B.item = int(item);
+    B.items.clear();
+    for (int i = 0; i < n; ++i)
+        for (int64_t b = 0; b < P[i].count; b += item)
+            B.items.push_back(WorkItem{i, int32_t(std::min<int64_t>(item, P[i].count - b)), P[i].first + b});
+    B.grid = int(std::min<int64_t>(B.grid, int64_t(B.items.size())));
+    return MCG_OK;
+}
+
+template <typename F>
+int run_batch(int n, mcg_scene* const* scenes, const HostProblem* P, double* const* outs,
+              mcg_stats* stats, const mcg_tune* tune) {
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(scenes[0]->device));
+    BatchPlan B;
+    int rc = plan_batch<F>(n, scenes, P, tune, B);
+    if (rc) return rc;
+    DevBuf<double> direct, diff;
+    DevBuf<unsigned long long> ctr;
+    DevBuf<int64_t> cdf;
+    DevBuf<KParams<F>> kp;
+    DevBuf<WorkItem> items;
+    DevBuf<unsigned int> item_ctr;
+    std::vector<int64_t> cdf_all;
+    std::vector<size_t> cdf_off(n);
+    for (int i = 0; i < n; ++i) {
+        cdf_off[i] = cdf_all.size();
+        cdf_all.insert(cdf_all.end(), P[i].emit_cdf.begin(), P[i].emit_cdf.end());
+    }
+    if ((rc = direct.alloc(B.n_direct))) return rc;
+    if ((rc = diff.alloc(B.n_diff))) return rc;
+    if ((rc = ctr.alloc(size_t(C_COUNT) * n))) return rc;
+    if ((rc = cdf.upload(cdf_all.data(), cdf_all.size()))) return rc;
+    if ((rc = items.upload(B.items.data(), B.items.size()))) return rc;
+    if ((rc = item_ctr.alloc(1))) return rc;
+    std::vector<KParams<F>> K(n);
+    for (int i = 0; i < n; ++i) {
+        K[i] = KParams<F>{};
+        fill_params<F>(scenes[i], P[i], K[i]);
+        K[i].emit_cdf = cdf.p + cdf_off[i];
+        K[i].direct = direct.p + B.off_direct[i];
+        K[i].diff = diff.p + B.off_diff[i];
+        K[i].counters = ctr.p + size_t(C_COUNT) * i;
+        K[i].lds_tally = B.tally ? 1 : 0;
+    }
+    if ((rc = kp.upload(K.data(), K.size()))) return rc;
+    HIP_TRY(hipMemset(direct.p, 0, direct.n * sizeof(double)));
+    HIP_TRY(hipMemset(diff.p, 0, diff.n * sizeof(double)));
+    HIP_TRY(hipMemset(ctr.p, 0, ctr.n * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(item_ctr.p, 0, sizeof(unsigned int)));
+    const void* fn = B.tables ? (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, true, true>)
+                                         : reinterpret_cast<const void*>(batch_kernel<F, true, false>))
+                              : (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, false, true>)
+                                         : reinterpret_cast<const void*>(batch_kernel<F, false, false>));
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, 0));
+    const int nitems = int(B.items.size());
+    if (B.tables && B.tally)
+        hipLaunchKernelGGL((batch_kernel<F, true, true>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
+    else if (B.tables)
+        hipLaunchKernelGGL((batch_kernel<F, true, false>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
+    else if (B.tally)
+        hipLaunchKernelGGL((batch_kernel<F, false, true>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
+    else
+        hipLaunchKernelGGL((batch_kernel<F, false, false>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<double> hd(direct.n), hf(diff.n);
+    std::vector<unsigned long long> hc(ctr.n);
+    HIP_TRY(hipMemcpy(hd.data(), direct.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hf.data(), diff.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hc.data(), ctr.p, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < n; ++i) {
+        finalise_field(scenes[i]->host, P[i], hd.data() + B.off_direct[i], hf.data() + B.off_diff[i], outs[i]);
+        const unsigned long long* c = hc.data() + size_t(C_COUNT) * i;
+        if (stats) {
+            stats[i].bundles = int64_t(c[C_BUNDLES]);
+            stats[i].steps = int64_t(c[C_STEPS]);
+            stats[i].escapes = int64_t(c[C_ESC]);
+            stats[i].absorbed = int64_t(c[C_ABS]);
+            stats[i].truncated = int64_t(c[C_TRUNC]);
+            stats[i].kernel_ms = ms;
+            stats[i].wall_ms = wall;
+        }
+        if (int64_t(c[C_BUNDLES]) != P[i].count)
+            return set_error(MCG_EDEVICE, "batch finished with bundles unaccounted for");
+    }
     return MCG_OK;
 }
 
@@ -985,6 +1286,34 @@ int mcg_scene_create(const mcg_material_spec* mat, const mcg_domain_spec* dom, m
     return MCG_OK;
 }
 
+int mcg_scene_plan(const mcg_scene* scene, const mcg_problem_spec* prob, mcg_problem_info* info) {
+    if (!scene || !prob || !info) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    mcg::HostProblem p;
+    int rc = mcg::build_problem(prob, scene->host, p);
+    if (rc) return rc;
+    info->nemit = p.nemit;
+    info->rows = p.rows;
+    info->cols = scene->host.cols;
+    info->maxloop = p.maxloop;
+    info->nemitter = int64_t(scene->host.emitters.size());
+    info->power = p.power;
+    info->weight_sum = p.weight_sum;
+    return MCG_OK;
+}
+
+int mcg_scene_tune(mcg_scene* scene, const mcg_tune* tune) {
+    if (!scene) return mcg::set_error(MCG_EINVAL, "NULL scene");
+    mcg_tune t = tune ? *tune : mcg_tune{0, 0, 0, 0, 0};
+    if (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256 && t.block != 512 &&
+        t.block != 1024)
+        return mcg::set_error(MCG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+    if (t.chunk < 0 || t.blocks_per_cu < 0 || t.per_thread < 0)
+        return mcg::set_error(MCG_EINVAL, "negative tuning value");
+    if (t.chunk != 0 && t.chunk < 64) return mcg::set_error(MCG_EINVAL, "chunk must be at least 64");
+    scene->tune = t;
+    return MCG_OK;
+}
+
 int mcg_scene_destroy(mcg_scene* scene) {
     delete scene;
     return MCG_OK;
@@ -1032,12 +1361,37 @@ int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, do
 int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
                     double* const* outs, const int64_t* out_lens, mcg_stats* stats,
                     const mcg_tune* tune) {
-    (void)tune;
     if (n < 0 || (n > 0 && (!scenes || !probs || !outs || !out_lens)))
         return mcg::set_error(MCG_EINVAL, "NULL argument");
+    if (n == 0) return MCG_OK;
+    std::vector<mcg::HostProblem> P(n);
+    std::vector<int> by_prec[2];
     for (int32_t i = 0; i < n; ++i) {
-        int rc = mcg_solve(scenes[i], &probs[i], outs[i], out_lens[i], nullptr, stats ? &stats[i] : nullptr);
+        if (!scenes[i] || !outs[i]) return mcg::set_error(MCG_EINVAL, "NULL scene or output");
+        int rc = mcg::build_problem(&probs[i], scenes[i]->host, P[i]);
         if (rc) return rc;
+        if (out_lens[i] < P[i].rows * scenes[i]->host.cols)
+            return mcg::set_error(MCG_ERANGE, "output buffer too small");
+        by_prec[P[i].precision == MCG_FP64 ? 1 : 0].push_back(i);
+    }
+    // one launch per precision present
+    for (int g = 0; g < 2; ++g) {
+        const std::vector<int>& idx = by_prec[g];
+        if (idx.empty()) continue;
+        std::vector<mcg_scene*> sc;
+        std::vector<mcg::HostProblem> pp;
+        std::vector<double*> oo;
+        std::vector<mcg_stats> ss(idx.size());
+        for (int i : idx) {
+            sc.push_back(scenes[i]);
+            pp.push_back(P[i]);
+            oo.push_back(outs[i]);
+        }
+        int rc = g == 1 ? mcg::run_batch<double>(int(idx.size()), sc.data(), pp.data(), oo.data(), ss.data(), tune)
+                        : mcg::run_batch<float>(int(idx.size()), sc.data(), pp.data(), oo.data(), ss.data(), tune);
+        if (rc) return rc;
+        if (stats)
+            for (size_t k = 0; k < idx.size(); ++k) stats[idx[k]] = ss[k];
     }
     return MCG_OK;
 }

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmcg.so")
 # every function include/mcg.h declares
 EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_problem_plan",
            "mcg_alias_build", "mcg_philox4x32_10", "mcg_scene_create", "mcg_scene_destroy",
+           "mcg_scene_plan", "mcg_scene_tune",
            "mcg_solve", "mcg_trace", "mcg_batch_solve")
 
 TRACE_WIDTH = 16
@@ -49,6 +50,8 @@ def lib():
         L.mcg_philox4x32_10.restype = None
         L.mcg_scene_create.argtypes = [MS, DS, C.POINTER(C.c_void_p)]
         L.mcg_scene_destroy.argtypes = [C.c_void_p]
+        L.mcg_scene_plan.argtypes = [C.c_void_p, PS, C.POINTER(abi.ProblemInfo)]
+        L.mcg_scene_tune.argtypes = [C.c_void_p, C.POINTER(abi.Tune)]
         L.mcg_solve.argtypes = [C.c_void_p, PS, C.POINTER(C.c_double), C.c_int64,
                                 C.POINTER(abi.Counts), C.POINTER(abi.Stats)]
         L.mcg_trace.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double), C.c_int64,
@@ -130,10 +133,19 @@ class Scene:
     def __exit__(self, *exc):
         self.close()
 
+    def set_tune(self, tune):
+        """Launch shape of this scene's single-solve kernel (None: defaults)."""
+        check(lib().mcg_scene_tune(self.handle, C.byref(tune) if tune is not None else None))
+
+    def plan(self, prob):
+        info = abi.ProblemInfo()
+        check(lib().mcg_scene_plan(self.handle, C.byref(prob), C.byref(info)))
+        return info
+
     def solve(self, prob, counts=False):
         """FieldProblem::solve on the GPU.  Returns (field rows x cols, stats
         dict, CountsBuffers or None)."""
-        info, _, _ = plan(self.mat, self.dom, prob)
+        info = self.plan(prob)
         out = np.zeros(info.rows * info.cols)
         nb = prob.bundle_count if prob.bundle_count else info.nemit - prob.first_bundle
         cb = abi.CountsBuffers(nb) if counts else None
@@ -149,4 +161,21 @@ class Scene:
         check(lib().mcg_trace(self.handle, C.byref(prob), int(bundle), abi.dptr(rec), cap,
                               C.byref(n)))
         return rec.reshape(cap, TRACE_WIDTH)[:n.value].copy()
+
+
+def batch_solve(scenes_, probs, tune=None):
+    """mcg_batch_solve: all solves in one persistent launch (per precision).
+    Returns (list of fields, list of stats dicts)."""
+    n = len(scenes_)
+    infos = [sc.plan(p) for sc, p in zip(scenes_, probs)]
+    outs = [np.zeros(i.rows * i.cols) for i in infos]
+    handles = (C.c_void_p * n)(*[sc.handle.value for sc in scenes_])
+    parr = (abi.ProblemSpec * n)(*probs)
+    optr = (C.POINTER(C.c_double) * n)(*[abi.dptr(o) for o in outs])
+    lens = (C.c_int64 * n)(*[o.size for o in outs])
+    stats = (abi.Stats * n)()
+    t = tune if tune is not None else abi.Tune(0, 0, 0, 0, 0)
+    check(lib().mcg_batch_solve(n, handles, parr, optr, lens, stats, C.byref(t)))
+    fields = [o.reshape(i.cols, i.rows).T.copy() for o, i in zip(outs, infos)]
+    return fields, [s.as_dict() for s in stats]
 

@@ -1,0 +1,36 @@
+"""Batched solves (mcg_batch_solve): one persistent launch over many solves
+with different materials, geometries, tally sizes and bundle counts must give
+each solve exactly what its own mcg_solve gives (same streams; the tallies
+differ only by FP64 summation order)."""
+import numpy as np
+import pytest
+
+from montecarlocpp_amd import abi, engine, materials, scenes, workloads
+from tests.helpers import rownorm_rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_FLIGHT])
+@pytest.mark.parametrize("tune", [None, abi.Tune(64, 1, 1000, 0, 0), abi.Tune(512, 1, 0, 1, -1)])
+def test_batch_equals_individual_solves(precision, tune):
+    cases = workloads.slab_batch(12, precision=precision)
+    cases = [(m, d, scenes.problem(min(p.nemit, 60_000), seed=p.seed, precision=precision))
+             for m, d, p in cases]
+    cases.append((materials.si_like(300.0), scenes.bar(4e-7, 1e-7, 1e-7, 30),
+                  scenes.problem(50_000, seed=99, precision=precision)))
+    cases.append((materials.si_like(250.0), scenes.film((2e-7, 6e-8, 2e-7), (8, 0, 0), 0.3),
+                  scenes.problem(20_000, tally=abi.CUMFLUX, size=5, maxscat=21, seed=98,
+                                 precision=precision)))
+    print("batch of", len(cases), "tune", tune and (tune.block, tune.chunk), flush=True)
+    sc = [engine.Scene(m, d) for m, d, _ in cases]
+    probs = [p for _, _, p in cases]
+    fb, sb = engine.batch_solve(sc, probs, tune)
+    for k, (s, p) in enumerate(zip(sc, probs)):
+        f1, s1, _ = s.solve(p)
+        assert sb[k]["steps"] == s1["steps"], k
+        assert sb[k]["bundles"] == s1["bundles"]
+        err = rownorm_rel(fb[k], f1)
+        assert np.all(err < 1e-10), (k, err)
+    for s in sc:
+        s.close()
