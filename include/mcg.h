@@ -265,6 +265,50 @@ int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec*
                     double* const* outs, const int64_t* out_lens,
                     mcg_stats* stats, const mcg_tune* tune);
 
+/* ---------------------------------------------------------------- serving
+ * Continuous batching.  A request is one field solve (its own material,
+ * domain and problem).  mcg_server_step admits queued requests in arrival
+ * order while their device memory fits the budget, runs one persistent batch
+ * launch over fair shares of every active request's remaining bundles (at
+ * most step_bundles in total), and retires the requests whose bundles are all
+ * done.  Tallies and counters of an admitted request stay on the GPU between
+ * steps.  Results equal mcg_solve's for the same problem, alone or mixed with
+ * other requests.  One host thread drives a server. */
+typedef struct mcg_server mcg_server;
+
+typedef struct mcg_server_stats {
+    int64_t queued, active;
+    uint64_t bytes_used, bytes_budget;
+    double kernel_ms;       /* device time of all steps so far          */
+    int64_t steps;          /* scatter-steps of retired requests        */
+} mcg_server_stats;
+
+int mcg_server_create(uint64_t hbm_budget_bytes, int64_t step_bundles, const mcg_tune* tune,
+                      mcg_server** out);
+int mcg_server_submit(mcg_server* server, const mcg_material_spec* mat,
+                      const mcg_domain_spec* dom, const mcg_problem_spec* prob,
+                      int64_t* request_id);
+int mcg_server_step(mcg_server* server, int32_t* active, int32_t* queued, int32_t* completed);
+/* MCG_OK and the result (then the request is forgotten), MCG_EAGAIN while it
+ * runs, MCG_ENOTFOUND for an unknown id.  stats.wall_ms = submit to done. */
+int mcg_server_poll(mcg_server* server, int64_t request_id, double* out, int64_t out_len,
+                    mcg_stats* stats);
+int mcg_server_cancel(mcg_server* server, int64_t request_id);
+int mcg_server_info(const mcg_server* server, mcg_server_stats* out);
+int mcg_server_destroy(mcg_server* server);
+
+/* ------------------------------------------------------------- autotuner
+ * Times candidate launch shapes (block 64-1024, sampling tables in LDS or
+ * not, wavefront grab size or batch work-item size) on a bounded sample of
+ * the problem's bundles and returns the fastest; mcg_autotune also installs
+ * it on the scene (mcg_scene_tune).  Only scheduling changes, never results.
+ * sample_bundles <= 0 means min(nemit, 2^20); sample_fraction <= 0 means 0.1.
+ * Phonons carried per thread is fixed at 1 in this build. */
+int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_bundles,
+                 mcg_tune* best, double* best_steps_per_s);
+int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
+                       double sample_fraction, mcg_tune* best, double* best_steps_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,3 +115,9 @@ class CountsBuffers:
         self.fate = np.zeros(n, np.uint32)
         self.struct = Counts(u32ptr(self.nscat), u32ptr(self.nbdry), u32ptr(self.nstep),
                              u32ptr(self.fate))
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int64), ("active", C.c_int64), ("bytes_used", C.c_uint64),
+                ("bytes_budget", C.c_uint64), ("kernel_ms", C.c_double), ("steps", C.c_int64)]
+

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -1036,28 +1038,19 @@ struct BatchPlan {
     int block = 256, grid = 0, chunk = 0, item = 0;
 };
 
+// LDS layout, block size and grid of a batch launch over these solves.
 template <typename F>
-int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_tune* tune,
+int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_tune* tune,
                BatchPlan& B) {
     const mcg_scene* s0 = scenes[0];
-    B.off_direct.resize(n);
-    B.off_diff.resize(n);
-    B.n_direct = B.n_diff = 0;
-    size_t lds_tables = 0, lds_plain = 0, lds_tally = 0, tally_bytes = 0;
-    int64_t total = 0;
+    size_t lds_tables = 0, lds_plain = 0, tally_bytes = 0;
     for (int i = 0; i < n; ++i) {
         const HostScene& h = scenes[i]->host;
         if (scenes[i]->device != s0->device) return set_error(MCG_EINVAL, "batch scenes on different devices");
-        B.off_direct[i] = B.n_direct;
-        B.off_diff[i] = B.n_diff;
-        B.n_direct += size_t(P[i].rows) * h.cols;
-        B.n_diff += size_t(P[i].rows) * (h.cols + 1);
         lds_tables = std::max(lds_tables, lds_bytes<F>(h, int(P[i].rows), true, false));
         lds_plain = std::max(lds_plain, lds_bytes<F>(h, int(P[i].rows), false, false));
-        lds_tally = std::max(lds_tally, lds_bytes<F>(h, int(P[i].rows), true, true));
         tally_bytes = std::max(tally_bytes, lds_bytes<F>(h, int(P[i].rows), false, true) -
                                                 lds_bytes<F>(h, int(P[i].rows), false, false));
-        total += P[i].count;
     }
     B.block = (tune && tune->block) ? tune->block : 512;
     if (B.block != 64 && B.block != 128 && B.block != 256 && B.block != 512 && B.block != 1024)
@@ -1068,12 +1061,32 @@ int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
     B.lds = base + (B.tally ? tally_bytes : 0);
     if (B.lds > s0->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     // occupancy: LDS per CU (the per-block limit is the CU's LDS on gfx950)
-    // and 16 waves per CU for this kernel's register use
+    // and at most 2048 threads per CU
     int per_cu = std::max<int>(1, int(s0->lds_limit / std::max<size_t>(B.lds, 1)));
     per_cu = std::min(per_cu, std::max(1, 2048 / B.block));
     if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, int(tune->blocks_per_cu));
     B.grid = s0->num_cu * per_cu;
     B.chunk = 64;
+    return MCG_OK;
+}
+
+template <typename F>
+int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_tune* tune,
+               BatchPlan& B) {
+    int rc = plan_shape<F>(n, scenes, P, tune, B);
+    if (rc) return rc;
+    B.off_direct.resize(n);
+    B.off_diff.resize(n);
+    B.n_direct = B.n_diff = 0;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const HostScene& h = scenes[i]->host;
+        B.off_direct[i] = B.n_direct;
+        B.off_diff[i] = B.n_diff;
+        B.n_direct += size_t(P[i].rows) * h.cols;
+        B.n_diff += size_t(P[i].rows) * (h.cols + 1);
+        total += P[i].count;
+    }
     // work items: each solve's range cut into items of about total/(8 grid)
     int64_t item = (tune && tune->chunk > 0) ? tune->chunk
                                              : std::max<int64_t>(2048, total / (int64_t(B.grid) * 8));
@@ -1084,6 +1097,28 @@ int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
         for (int64_t b = 0; b < P[i].count; b += item)
             B.items.push_back(WorkItem{i, int32_t(std::min<int64_t>(item, P[i].count - b)), P[i].first + b});
     B.grid = int(std::min<int64_t>(B.grid, int64_t(B.items.size())));
+    return MCG_OK;
+}
+
+// Enqueue one batch launch on `stream`.
+template <typename F>
+int launch_batch(const BatchPlan& B, const KParams<F>* kp, const WorkItem* items, int nitems,
+                 unsigned int* item_ctr, hipStream_t stream) {
+    const void* fn = B.tables ? (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, true, true>)
+                                         : reinterpret_cast<const void*>(batch_kernel<F, true, false>))
+                              : (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, false, true>)
+                                         : reinterpret_cast<const void*>(batch_kernel<F, false, false>));
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
+    const int grid = std::max(1, std::min(B.grid, nitems));
+    if (B.tables && B.tally)
+        hipLaunchKernelGGL((batch_kernel<F, true, true>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
+    else if (B.tables)
+        hipLaunchKernelGGL((batch_kernel<F, true, false>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
+    else if (B.tally)
+        hipLaunchKernelGGL((batch_kernel<F, false, true>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
+    else
+        hipLaunchKernelGGL((batch_kernel<F, false, false>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
+    HIP_TRY(hipGetLastError());
     return MCG_OK;
 }
 
@@ -1128,25 +1163,11 @@ int run_batch(int n, mcg_scene* const* scenes, const HostProblem* P, double* con
     HIP_TRY(hipMemset(diff.p, 0, diff.n * sizeof(double)));
     HIP_TRY(hipMemset(ctr.p, 0, ctr.n * sizeof(unsigned long long)));
     HIP_TRY(hipMemset(item_ctr.p, 0, sizeof(unsigned int)));
-    const void* fn = B.tables ? (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, true, true>)
-                                         : reinterpret_cast<const void*>(batch_kernel<F, true, false>))
-                              : (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, false, true>)
-                                         : reinterpret_cast<const void*>(batch_kernel<F, false, false>));
-    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, 0));
-    const int nitems = int(B.items.size());
-    if (B.tables && B.tally)
-        hipLaunchKernelGGL((batch_kernel<F, true, true>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
-    else if (B.tables)
-        hipLaunchKernelGGL((batch_kernel<F, true, false>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
-    else if (B.tally)
-        hipLaunchKernelGGL((batch_kernel<F, false, true>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
-    else
-        hipLaunchKernelGGL((batch_kernel<F, false, false>), dim3(B.grid), dim3(B.block), B.lds, 0, kp.p, items.p, nitems, item_ctr.p, B.chunk);
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_batch<F>(B, kp.p, items.p, int(B.items.size()), item_ctr.p, 0))) return rc;
     HIP_TRY(hipEventRecord(e1, 0));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1173,6 +1194,192 @@ int run_batch(int n, mcg_scene* const* scenes, const HostProblem* P, double* con
         }
         if (int64_t(c[C_BUNDLES]) != P[i].count)
             return set_error(MCG_EDEVICE, "batch finished with bundles unaccounted for");
+    }
+    return MCG_OK;
+}
+
+}  // namespace
+}  // namespace mcg
+
+
+namespace mcg {
+// Device side of a scene whose host part is built: query the current GPU and
+// upload geometry (both precisions), tables and Inter candidates.
+int scene_upload(mcg_scene* sc) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return set_error(MCG_EDEVICE, "no HIP device available");
+    HIP_TRY(hipGetDevice(&sc->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, sc->device));
+    sc->num_cu = prop.multiProcessorCount;
+    sc->lds_limit = prop.sharedMemPerBlock;
+    int rc;
+    if ((rc = upload_scene(sc->host, sc->d64))) return rc;
+    if ((rc = upload_scene(sc->host, sc->d32))) return rc;
+    if ((rc = sc->scat_alias.upload(sc->host.scat_alias.data(), sc->host.scat_alias.size()))) return rc;
+    if ((rc = sc->flux_alias.upload(sc->host.flux_alias.data(), sc->host.flux_alias.size()))) return rc;
+    if ((rc = sc->inter_list.upload(sc->host.inter_list.data(), sc->host.inter_list.size()))) return rc;
+    return MCG_OK;
+}
+
+// Device bytes a scene holds once uploaded.
+size_t scene_bytes(const HostScene& h) {
+    const size_t modes = h.mode_cell.size();
+    return h.planes.size() * (sizeof(DevPlane<double>) + sizeof(DevPlane<float>)) +
+           h.sdoms.size() * (sizeof(DevSdom<double>) + sizeof(DevSdom<float>)) +
+           h.emitters.size() * (sizeof(DevEmitter<double>) + sizeof(DevEmitter<float>)) +
+           h.pores.size() * (sizeof(DevPore<double>) + sizeof(DevPore<float>)) +
+           modes * (2 * 8 + 2 * 4 + 2 * 8) + h.inter_list.size() * 4;
+}
+}  // namespace mcg
+
+
+// ====================================================================== serving
+// Continuous batching: requests are admitted as device memory allows, their
+// tallies and counters stay on the GPU across steps, and every step packs
+// work items of all active requests (fair shares of a per-step bundle budget)
+// into one persistent batch launch per precision.  Bundles never straddle
+// steps: an item runs its bundles to termination inside the step.
+namespace mcg {
+
+struct Request {
+    int64_t id = 0;
+    std::unique_ptr<mcg_scene> scene;
+    HostProblem P;
+    int64_t cursor = 0;     // bundles handed out (offset from P.first)
+    int64_t done = 0;       // bundles finished
+    size_t bytes = 0;       // device memory held while admitted
+    DevBuf<double> direct, diff;
+    DevBuf<unsigned long long> ctr;
+    DevBuf<int64_t> cdf;
+    int state = 0;          // 0 queued, 1 active, 2 done
+    std::vector<double> result;
+    mcg_stats stats{};
+    std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+}  // namespace mcg
+
+struct mcg_server {
+    size_t budget = 0, used = 0;
+    int64_t step_bundles = 0;
+    mcg_tune tune{0, 0, 0, 0, 0};
+    int64_t next_id = 1;
+    std::deque<mcg::Request*> queue;
+    std::vector<mcg::Request*> active;
+    std::map<int64_t, std::unique_ptr<mcg::Request>> all;
+    hipStream_t stream = nullptr;
+    mcg::DevBuf<mcg::KParams<float>> kp32;
+    mcg::DevBuf<mcg::KParams<double>> kp64;
+    mcg::DevBuf<mcg::WorkItem> it32, it64;
+    mcg::DevBuf<unsigned int> ictr;
+    double kernel_ms_total = 0.0;
+    int64_t steps_total = 0;
+    ~mcg_server() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace mcg {
+namespace {
+
+int admit(mcg_server* sv, Request* r) {
+    int rc = scene_upload(r->scene.get());
+    if (rc) return rc;
+    const size_t nd = size_t(r->P.rows) * r->scene->host.cols;
+    const size_t nf = size_t(r->P.rows) * (r->scene->host.cols + 1);
+    if ((rc = r->direct.alloc(nd))) return rc;
+    if ((rc = r->diff.alloc(nf))) return rc;
+    if ((rc = r->ctr.alloc(C_COUNT))) return rc;
+    if ((rc = r->cdf.upload(r->P.emit_cdf.data(), r->P.emit_cdf.size()))) return rc;
+    HIP_TRY(hipMemsetAsync(r->direct.p, 0, nd * sizeof(double), sv->stream));
+    HIP_TRY(hipMemsetAsync(r->diff.p, 0, nf * sizeof(double), sv->stream));
+    HIP_TRY(hipMemsetAsync(r->ctr.p, 0, C_COUNT * sizeof(unsigned long long), sv->stream));
+    r->state = 1;
+    sv->used += r->bytes;
+    sv->active.push_back(r);
+    return MCG_OK;
+}
+
+void release(mcg_server* sv, Request* r) {
+    r->direct.alloc(0);
+    r->diff.alloc(0);
+    r->ctr.alloc(0);
+    r->cdf.alloc(0);
+    r->scene.reset();
+    sv->used -= std::min(sv->used, r->bytes);
+}
+
+int retire(mcg_server* sv, Request* r) {
+    std::vector<double> hd(r->direct.n), hf(r->diff.n);
+    unsigned long long hc[C_COUNT];
+    HIP_TRY(hipMemcpy(hd.data(), r->direct.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hf.data(), r->diff.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hc, r->ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
+    r->result.assign(size_t(r->P.rows * r->scene->host.cols), 0.0);
+    finalise_field(r->scene->host, r->P, hd.data(), hf.data(), r->result.data());
+    r->stats.bundles = int64_t(hc[C_BUNDLES]);
+    r->stats.steps = int64_t(hc[C_STEPS]);
+    r->stats.escapes = int64_t(hc[C_ESC]);
+    r->stats.absorbed = int64_t(hc[C_ABS]);
+    r->stats.truncated = int64_t(hc[C_TRUNC]);
+    r->t_done = std::chrono::steady_clock::now();
+    r->stats.wall_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    r->state = 2;
+    release(sv, r);
+    if (r->stats.bundles != r->P.count) return set_error(MCG_EDEVICE, "request finished with bundles unaccounted for");
+    return MCG_OK;
+}
+
+// Launch one precision group of the step; `reqs` are the requests with work.
+template <typename F>
+int step_group(mcg_server* sv, const std::vector<Request*>& reqs, const std::vector<int64_t>& take,
+               DevBuf<KParams<F>>& kpbuf, DevBuf<WorkItem>& itbuf, unsigned int* ictr, float* ms) {
+    const int n = int(reqs.size());
+    std::vector<mcg_scene*> sc(n);
+    std::vector<HostProblem> P(n);
+    for (int i = 0; i < n; ++i) {
+        sc[i] = reqs[i]->scene.get();
+        P[i] = reqs[i]->P;
+    }
+    BatchPlan B;
+    int rc = plan_shape<F>(n, sc.data(), P.data(), &sv->tune, B);
+    if (rc) return rc;
+    const int64_t item = sv->tune.chunk > 0 ? sv->tune.chunk : 8192;
+    std::vector<WorkItem> items;
+    std::vector<KParams<F>> K(n);
+    for (int i = 0; i < n; ++i) {
+        Request* r = reqs[i];
+        K[i] = KParams<F>{};
+        fill_params<F>(sc[i], r->P, K[i]);
+        K[i].emit_cdf = r->cdf.p;
+        K[i].direct = r->direct.p;
+        K[i].diff = r->diff.p;
+        K[i].counters = r->ctr.p;
+        K[i].lds_tally = B.tally ? 1 : 0;
+        for (int64_t b = 0; b < take[i]; b += item)
+            items.push_back(WorkItem{i, int32_t(std::min<int64_t>(item, take[i] - b)),
+                                     r->P.first + r->cursor + b});
+    }
+    if (kpbuf.n < size_t(n) && (rc = kpbuf.alloc(size_t(n) * 2))) return rc;
+    if (itbuf.n < items.size() && (rc = itbuf.alloc(items.size() * 2))) return rc;
+    HIP_TRY(hipMemcpyAsync(kpbuf.p, K.data(), K.size() * sizeof(KParams<F>), hipMemcpyHostToDevice, sv->stream));
+    HIP_TRY(hipMemcpyAsync(itbuf.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, sv->stream));
+    HIP_TRY(hipMemsetAsync(ictr, 0, sizeof(unsigned int), sv->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, sv->stream));
+    if ((rc = launch_batch<F>(B, kpbuf.p, itbuf.p, int(items.size()), ictr, sv->stream))) return rc;
+    HIP_TRY(hipEventRecord(e1, sv->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int i = 0; i < n; ++i) {
+        reqs[i]->cursor += take[i];
+        reqs[i]->done += take[i];
     }
     return MCG_OK;
 }
@@ -1268,20 +1475,8 @@ int mcg_scene_create(const mcg_material_spec* mat, const mcg_domain_spec* dom, m
     if (!sc) return mcg::set_error(MCG_ENOMEM, "out of host memory");
     int rc = mcg::build_material(mat, sc->host);
     if (!rc) rc = mcg::build_domain(dom, sc->host);
+    if (!rc) rc = mcg::scene_upload(sc.get());
     if (rc) return rc;
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || ndev == 0) return mcg::set_error(MCG_EDEVICE, "no HIP device available");
-    HIP_TRY(hipGetDevice(&sc->device));
-    hipDeviceProp_t prop;
-    HIP_TRY(hipGetDeviceProperties(&prop, sc->device));
-    sc->num_cu = prop.multiProcessorCount;
-    sc->lds_limit = prop.sharedMemPerBlock;
-    if ((rc = mcg::upload_scene(sc->host, sc->d64))) return rc;
-    if ((rc = mcg::upload_scene(sc->host, sc->d32))) return rc;
-    if ((rc = sc->scat_alias.upload(sc->host.scat_alias.data(), sc->host.scat_alias.size()))) return rc;
-    if ((rc = sc->flux_alias.upload(sc->host.flux_alias.data(), sc->host.flux_alias.size()))) return rc;
-    if ((rc = sc->inter_list.upload(sc->host.inter_list.data(), sc->host.inter_list.size()))) return rc;
     *out = sc.release();
     return MCG_OK;
 }
@@ -1393,6 +1588,246 @@ int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec*
         if (stats)
             for (size_t k = 0; k < idx.size(); ++k) stats[idx[k]] = ss[k];
     }
+    return MCG_OK;
+}
+
+int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_bundles,
+                 mcg_tune* best, double* best_steps_per_s) {
+    if (!scene || !prob) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    mcg::HostProblem p;
+    int rc = mcg::build_problem(prob, scene->host, p);
+    if (rc) return rc;
+    const int64_t n = sample_bundles > 0 ? std::min(sample_bundles, p.count)
+                                         : std::min<int64_t>(p.count, int64_t(1) << 20);
+    p.count = n;
+    std::vector<double> out(size_t(p.rows * scene->host.cols));
+    const mcg_tune saved = scene->tune;
+    mcg_tune top = saved;
+    double top_rate = -1.0;
+    static const int blocks[] = {64, 128, 256, 512, 1024};
+    static const int tables[] = {1, -1};
+    static const int chunks[] = {64, 256, 1024};
+    for (int b : blocks)
+        for (int t : tables)
+            for (int c : chunks) {
+                scene->tune = mcg_tune{b, 1, c, 0, t};
+                mcg_stats st{};
+                double ms = 1e300;
+                for (int rep = 0; rep < 2; ++rep) {   // first run warms up the shape
+                    rc = p.precision == MCG_FP64 ? mcg::run_solve<double>(scene, p, out.data(), nullptr, &st)
+                                                 : mcg::run_solve<float>(scene, p, out.data(), nullptr, &st);
+                    if (rc) {
+                        scene->tune = saved;
+                        return rc;
+                    }
+                    ms = std::min(ms, st.kernel_ms);
+                }
+                const double rate = double(st.steps) / (ms * 1e-3);
+                if (rate > top_rate) {
+                    top_rate = rate;
+                    top = scene->tune;
+                }
+            }
+    scene->tune = top;
+    if (best) *best = top;
+    if (best_steps_per_s) *best_steps_per_s = top_rate;
+    return MCG_OK;
+}
+
+int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
+                       double sample_fraction, mcg_tune* best, double* best_steps_per_s) {
+    if (n <= 0 || !scenes || !probs) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    const double frac = sample_fraction > 0.0 ? std::min(sample_fraction, 1.0) : 0.1;
+    std::vector<mcg::HostProblem> P(n);
+    std::vector<mcg_scene*> sc(scenes, scenes + n);
+    std::vector<std::vector<double>> outs(n);
+    std::vector<double*> optr(n);
+    int prec = -1;
+    for (int i = 0; i < n; ++i) {
+        int rc = mcg::build_problem(&probs[i], scenes[i]->host, P[i]);
+        if (rc) return rc;
+        if (prec >= 0 && P[i].precision != prec)
+            return mcg::set_error(MCG_EINVAL, "autotune a batch of one precision at a time");
+        prec = P[i].precision;
+        P[i].count = std::max<int64_t>(std::min<int64_t>(P[i].count, 256), int64_t(double(P[i].count) * frac));
+        outs[i].resize(size_t(P[i].rows * scenes[i]->host.cols));
+        optr[i] = outs[i].data();
+    }
+    std::vector<mcg_stats> st(n);
+    mcg_tune top{0, 1, 0, 0, 0};
+    double top_rate = -1.0;
+    static const int blocks[] = {128, 256, 512, 1024};
+    static const int tables[] = {1, -1};
+    static const int items[] = {0, 4096, 16384, 65536};
+    for (int b : blocks)
+        for (int t : tables)
+            for (int it : items) {
+                const mcg_tune cand{b, 1, it, 0, t};
+                double ms = 1e300;
+                int64_t steps = 0;
+                for (int rep = 0; rep < 2; ++rep) {
+                    int rc = prec == MCG_FP64
+                                 ? mcg::run_batch<double>(n, sc.data(), P.data(), optr.data(), st.data(), &cand)
+                                 : mcg::run_batch<float>(n, sc.data(), P.data(), optr.data(), st.data(), &cand);
+                    if (rc) return rc;
+                    ms = std::min(ms, st[0].kernel_ms);
+                    steps = 0;
+                    for (const mcg_stats& x : st) steps += x.steps;
+                }
+                const double rate = double(steps) / (ms * 1e-3);
+                if (rate > top_rate) {
+                    top_rate = rate;
+                    top = cand;
+                }
+            }
+    if (best) *best = top;
+    if (best_steps_per_s) *best_steps_per_s = top_rate;
+    return MCG_OK;
+}
+
+
+int mcg_server_create(uint64_t hbm_budget_bytes, int64_t step_bundles, const mcg_tune* tune,
+                      mcg_server** out) {
+    if (!out) return mcg::set_error(MCG_EINVAL, "NULL output");
+    *out = nullptr;
+    std::unique_ptr<mcg_server> sv(new (std::nothrow) mcg_server());
+    if (!sv) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return mcg::set_error(MCG_EDEVICE, "no HIP device available");
+    sv->budget = hbm_budget_bytes > 0 ? size_t(hbm_budget_bytes) : size_t(200) << 30;
+    sv->step_bundles = step_bundles > 0 ? step_bundles : (int64_t(1) << 22);
+    if (tune) sv->tune = *tune;
+    HIP_TRY(hipStreamCreate(&sv->stream));
+    int rc = sv->ictr.alloc(2);
+    if (rc) return rc;
+    *out = sv.release();
+    return MCG_OK;
+}
+
+int mcg_server_submit(mcg_server* sv, const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                      const mcg_problem_spec* prob, int64_t* request_id) {
+    if (!sv || !request_id) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    std::unique_ptr<mcg::Request> r(new (std::nothrow) mcg::Request());
+    if (!r) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+    r->scene.reset(new (std::nothrow) mcg_scene());
+    if (!r->scene) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+    int rc = mcg::build_material(mat, r->scene->host);
+    if (!rc) rc = mcg::build_domain(dom, r->scene->host);
+    if (!rc) rc = mcg::build_problem(prob, r->scene->host, r->P);
+    if (rc) return rc;
+    r->bytes = mcg::scene_bytes(r->scene->host) +
+               sizeof(double) * size_t(r->P.rows) * (2 * r->scene->host.cols + 1) +
+               sizeof(unsigned long long) * mcg::C_COUNT + sizeof(int64_t) * r->P.emit_cdf.size();
+    if (r->bytes > sv->budget) return mcg::set_error(MCG_ENOMEM, "request exceeds the device memory budget");
+    r->id = sv->next_id++;
+    r->t_submit = std::chrono::steady_clock::now();
+    *request_id = r->id;
+    sv->queue.push_back(r.get());
+    sv->all[r->id] = std::move(r);
+    return MCG_OK;
+}
+
+int mcg_server_step(mcg_server* sv, int32_t* active, int32_t* queued, int32_t* completed) {
+    if (!sv) return mcg::set_error(MCG_EINVAL, "NULL server");
+    int rc;
+    // admission in arrival order while the budget allows
+    while (!sv->queue.empty() && sv->used + sv->queue.front()->bytes <= sv->budget) {
+        mcg::Request* r = sv->queue.front();
+        sv->queue.pop_front();
+        if ((rc = mcg::admit(sv, r))) return rc;
+    }
+    // fair share of the step's bundle budget
+    std::vector<mcg::Request*> g32, g64;
+    std::vector<int64_t> t32, t64;
+    int with_work = 0;
+    for (mcg::Request* r : sv->active)
+        if (r->cursor < r->P.count) ++with_work;
+    const int64_t share = with_work ? std::max<int64_t>(1024, sv->step_bundles / with_work) : 0;
+    for (mcg::Request* r : sv->active) {
+        const int64_t take = std::min(share, r->P.count - r->cursor);
+        if (take <= 0) continue;
+        if (r->P.precision == MCG_FP64) {
+            g64.push_back(r);
+            t64.push_back(take);
+        } else {
+            g32.push_back(r);
+            t32.push_back(take);
+        }
+    }
+    float ms = 0.f;
+    if (!g32.empty()) {
+        if ((rc = mcg::step_group<float>(sv, g32, t32, sv->kp32, sv->it32, sv->ictr.p, &ms))) return rc;
+        sv->kernel_ms_total += ms;
+    }
+    if (!g64.empty()) {
+        if ((rc = mcg::step_group<double>(sv, g64, t64, sv->kp64, sv->it64, sv->ictr.p + 1, &ms))) return rc;
+        sv->kernel_ms_total += ms;
+    }
+    // retire finished requests
+    int ndone = 0;
+    std::vector<mcg::Request*> still;
+    for (mcg::Request* r : sv->active) {
+        if (r->done >= r->P.count) {
+            if ((rc = mcg::retire(sv, r))) return rc;
+            sv->steps_total += r->stats.steps;
+            ++ndone;
+        } else {
+            still.push_back(r);
+        }
+    }
+    sv->active.swap(still);
+    if (active) *active = int32_t(sv->active.size());
+    if (queued) *queued = int32_t(sv->queue.size());
+    if (completed) *completed = ndone;
+    return MCG_OK;
+}
+
+int mcg_server_poll(mcg_server* sv, int64_t id, double* out, int64_t out_len, mcg_stats* stats) {
+    if (!sv) return mcg::set_error(MCG_EINVAL, "NULL server");
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return mcg::set_error(MCG_ENOTFOUND, "unknown request id");
+    mcg::Request* r = it->second.get();
+    if (r->state != 2) return MCG_EAGAIN;
+    if (out) {
+        if (out_len < int64_t(r->result.size())) return mcg::set_error(MCG_ERANGE, "output buffer too small");
+        std::memcpy(out, r->result.data(), r->result.size() * sizeof(double));
+    }
+    if (stats) *stats = r->stats;
+    sv->all.erase(it);
+    return MCG_OK;
+}
+
+int mcg_server_cancel(mcg_server* sv, int64_t id) {
+    if (!sv) return mcg::set_error(MCG_EINVAL, "NULL server");
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return mcg::set_error(MCG_ENOTFOUND, "unknown request id");
+    mcg::Request* r = it->second.get();
+    if (r->state == 0) {
+        sv->queue.erase(std::remove(sv->queue.begin(), sv->queue.end(), r), sv->queue.end());
+    } else if (r->state == 1) {
+        sv->active.erase(std::remove(sv->active.begin(), sv->active.end(), r), sv->active.end());
+        mcg::release(sv, r);
+    }
+    sv->all.erase(it);
+    return MCG_OK;
+}
+
+int mcg_server_info(const mcg_server* sv, mcg_server_stats* out) {
+    if (!sv || !out) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    out->queued = int64_t(sv->queue.size());
+    out->active = int64_t(sv->active.size());
+    out->bytes_used = uint64_t(sv->used);
+    out->bytes_budget = uint64_t(sv->budget);
+    out->kernel_ms = sv->kernel_ms_total;
+    out->steps = sv->steps_total;
+    return MCG_OK;
+}
+
+int mcg_server_destroy(mcg_server* sv) {
+    if (!sv) return MCG_OK;
+    for (mcg::Request* r : sv->active) mcg::release(sv, r);
+    delete sv;
     return MCG_OK;
 }
 

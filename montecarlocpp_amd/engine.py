@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmcg.so")
 # every function include/mcg.h declares
 EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_problem_plan",
            "mcg_alias_build", "mcg_philox4x32_10", "mcg_scene_create", "mcg_scene_destroy",
-           "mcg_scene_plan", "mcg_scene_tune",
+           "mcg_scene_plan", "mcg_scene_tune", "mcg_autotune", "mcg_autotune_batch",
+           "mcg_server_create", "mcg_server_submit", "mcg_server_step", "mcg_server_poll",
+           "mcg_server_cancel", "mcg_server_info", "mcg_server_destroy",
            "mcg_solve", "mcg_trace", "mcg_batch_solve")
 
 TRACE_WIDTH = 16
@@ -52,6 +54,20 @@ def lib():
         L.mcg_scene_destroy.argtypes = [C.c_void_p]
         L.mcg_scene_plan.argtypes = [C.c_void_p, PS, C.POINTER(abi.ProblemInfo)]
         L.mcg_scene_tune.argtypes = [C.c_void_p, C.POINTER(abi.Tune)]
+        L.mcg_server_create.argtypes = [C.c_uint64, C.c_int64, C.POINTER(abi.Tune),
+                                        C.POINTER(C.c_void_p)]
+        L.mcg_server_submit.argtypes = [C.c_void_p, MS, DS, PS, C.POINTER(C.c_int64)]
+        L.mcg_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]
+        L.mcg_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_int64,
+                                      C.POINTER(abi.Stats)]
+        L.mcg_server_cancel.argtypes = [C.c_void_p, C.c_int64]
+        L.mcg_server_info.argtypes = [C.c_void_p, C.POINTER(abi.ServerStats)]
+        L.mcg_server_destroy.argtypes = [C.c_void_p]
+        L.mcg_autotune.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(abi.Tune),
+                                   C.POINTER(C.c_double)]
+        L.mcg_autotune_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p), PS, C.c_double,
+                                         C.POINTER(abi.Tune), C.POINTER(C.c_double)]
         L.mcg_solve.argtypes = [C.c_void_p, PS, C.POINTER(C.c_double), C.c_int64,
                                 C.POINTER(abi.Counts), C.POINTER(abi.Stats)]
         L.mcg_trace.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double), C.c_int64,
@@ -137,6 +153,14 @@ class Scene:
         """Launch shape of this scene's single-solve kernel (None: defaults)."""
         check(lib().mcg_scene_tune(self.handle, C.byref(tune) if tune is not None else None))
 
+    def autotune(self, prob, sample_bundles=0):
+        """mcg_autotune: pick and install the fastest launch shape."""
+        t = abi.Tune()
+        r = C.c_double()
+        check(lib().mcg_autotune(self.handle, C.byref(prob), int(sample_bundles), C.byref(t),
+                                 C.byref(r)))
+        return t, r.value
+
     def plan(self, prob):
         info = abi.ProblemInfo()
         check(lib().mcg_scene_plan(self.handle, C.byref(prob), C.byref(info)))
@@ -178,4 +202,79 @@ def batch_solve(scenes_, probs, tune=None):
     check(lib().mcg_batch_solve(n, handles, parr, optr, lens, stats, C.byref(t)))
     fields = [o.reshape(i.cols, i.rows).T.copy() for o, i in zip(outs, infos)]
     return fields, [s.as_dict() for s in stats]
+
+
+def autotune_batch(scenes_, probs, sample_fraction=0.1):
+    """mcg_autotune_batch: the fastest batch launch shape on a sample."""
+    n = len(scenes_)
+    handles = (C.c_void_p * n)(*[sc.handle.value for sc in scenes_])
+    parr = (abi.ProblemSpec * n)(*probs)
+    t = abi.Tune()
+    r = C.c_double()
+    check(lib().mcg_autotune_batch(n, handles, parr, sample_fraction, C.byref(t), C.byref(r)))
+    return t, r.value
+
+
+class Server:
+    """Continuous-batching request server (mcg_server_*)."""
+
+    def __init__(self, budget_bytes=200 << 30, step_bundles=1 << 22, tune=None):
+        h = C.c_void_p()
+        check(lib().mcg_server_create(int(budget_bytes), int(step_bundles),
+                                      C.byref(tune) if tune is not None else None, C.byref(h)))
+        self.handle = h
+        self._keep = {}          # request id -> (mat, dom, rows, cols) kept alive until polled
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().mcg_server_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, mat, dom, prob):
+        info, _, _ = plan(mat, dom, prob)
+        rid = C.c_int64()
+        check(lib().mcg_server_submit(self.handle, C.byref(mat.spec), C.byref(dom.spec()),
+                                      C.byref(prob), C.byref(rid)))
+        self._keep[rid.value] = (mat, dom, info.rows, info.cols)
+        return rid.value
+
+    def step(self):
+        a, q, d = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().mcg_server_step(self.handle, C.byref(a), C.byref(q), C.byref(d)))
+        return a.value, q.value, d.value
+
+    def poll(self, rid):
+        """(field, stats) when done, None while running."""
+        if rid not in self._keep:      # unknown or already collected: let the server say so
+            check(lib().mcg_server_poll(self.handle, rid, None, 0, None))
+        _, _, rows, cols = self._keep[rid]
+        out = np.zeros(rows * cols)
+        st = abi.Stats()
+        rc = lib().mcg_server_poll(self.handle, rid, abi.dptr(out), out.size, C.byref(st))
+        if rc == abi.MCG_EAGAIN:
+            return None
+        check(rc)
+        del self._keep[rid]
+        return out.reshape(cols, rows).T.copy(), st.as_dict()
+
+    def cancel(self, rid):
+        check(lib().mcg_server_cancel(self.handle, rid))
+        self._keep.pop(rid, None)
+
+    def info(self):
+        s = abi.ServerStats()
+        check(lib().mcg_server_info(self.handle, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in s._fields_}
 

@@ -1,0 +1,130 @@
+"""Open-loop serving of field solves on one MI355X.
+
+Requests arrive as a Poisson stream of rate lam (solves/s).  Request id r
+draws, from the Philox block (counter (r, 0, 0, 0), key (0x5EED0004, 0)) as
+uniforms u0..u3: its family (the slab, the bar, or a random slab of the
+batched workload), its bundle count N = 10^(log10 nmin + (log10 nmax -
+log10 nmin) u1) and its T_ref = 100 + 400 u2 K; its seed is r.  Requests are
+submitted to an mcg_server as their arrival time passes and the server is
+stepped continuously; a request's latency runs from its scheduled arrival to
+the step that finished it.
+
+    python -m montecarlocpp_amd.serve --lam 100 --requests 300
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import abi, engine, materials, scenes, workloads
+
+REQUEST_KEY = (0x5EED0004, 0)
+FAMILIES = ("slab", "bar", "random_slab")
+
+
+class RequestFactory:
+    def __init__(self, nmin=1e4, nmax=1e8, precision=abi.FP32_FLIGHT, families=FAMILIES):
+        self.nmin, self.nmax, self.precision = nmin, nmax, precision
+        self.families = families
+        self.base = materials.si_like(300.0)
+        self._mat_cache = {}
+
+    def material(self, T):
+        key = round(T, 6)
+        m = self._mat_cache.get(key)
+        if m is None:
+            m = self.base.at_temperature(T)
+            if len(self._mat_cache) < 4096:
+                self._mat_cache[key] = m
+        return m
+
+    def make(self, rid):
+        w = engine.philox([rid, 0, 0, 0], list(REQUEST_KEY))
+        u = [x / 2.0 ** 32 for x in w]
+        fam = self.families[min(int(u[0] * len(self.families)), len(self.families) - 1)]
+        lo, hi = math.log10(self.nmin), math.log10(self.nmax)
+        n = int(round(10.0 ** (lo + (hi - lo) * u[1])))
+        T = 100.0 + 400.0 * u[2]
+        if fam == "slab":
+            dom = scenes.slab(100e-9, 50)
+        elif fam == "bar":
+            dom = scenes.bar(1e-6, 1e-7, 1e-7, 100, 10.0, -10.0)
+        else:
+            prm = workloads.slab_batch_params(1)[0]
+            L = 10.0 ** (-8.0 + 3.0 * u[3])
+            bins = int(round(20 + 180 * math.log10(L / 1e-8) / 3))
+            dom = scenes.slab(L, bins, side=abi.PARTIAL, specularity=u[3])
+            del prm
+        prob = scenes.problem(n, seed=rid, precision=self.precision)
+        return fam, self.material(T), dom, prob
+
+
+def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 22, verbose=False):
+    """Serve nreq Poisson arrivals at rate lam; returns a metrics dict."""
+    factory = factory or RequestFactory()
+    own = server is None
+    server = server or engine.Server(step_bundles=step_bundles)
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=nreq))
+    reqs = [factory.make(r + 1) for r in range(nreq)]   # built before the clock starts
+    lat = np.full(nreq, np.nan)
+    steps = np.zeros(nreq, np.int64)
+    outstanding = {}
+    k = 0
+    done = 0
+    t0 = time.perf_counter()
+    while done < nreq:
+        now = time.perf_counter() - t0
+        while k < nreq and arrivals[k] <= now:
+            _, mat, dom, prob = reqs[k]
+            outstanding[server.submit(mat, dom, prob)] = k
+            k += 1
+        if outstanding:
+            _, _, ncomp = server.step()
+            if ncomp:
+                tnow = time.perf_counter() - t0
+                for rid in list(outstanding):
+                    res = server.poll(rid)
+                    if res is not None:
+                        i = outstanding.pop(rid)
+                        lat[i] = tnow - arrivals[i]
+                        steps[i] = res[1]["steps"]
+                        done += 1
+        elif k < nreq:
+            time.sleep(max(0.0, min(arrivals[k] - now, 0.01)))
+    elapsed = time.perf_counter() - t0
+    info = server.info()
+    if own:
+        server.close()
+    out = {
+        "lambda_per_s": lam,
+        "requests": nreq,
+        "elapsed_s": elapsed,
+        "solves_per_s": nreq / elapsed,
+        "scatter_steps_per_s": float(steps.sum()) / elapsed,
+        "gpu_busy_frac": info["kernel_ms"] * 1e-3 / elapsed,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_mean_s": float(np.mean(lat)),
+    }
+    if verbose:
+        print(json.dumps(out), flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lam", type=float, nargs="+", default=[10, 25, 50, 100, 200, 500])
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--nmax", type=float, default=1e8)
+    args = ap.parse_args()
+    fac = RequestFactory(nmax=args.nmax)
+    with engine.Server() as sv:
+        for lam in args.lam:
+            run(lam, args.requests, server=sv, factory=fac, verbose=True)
+
+
+if __name__ == "__main__":
+    main()

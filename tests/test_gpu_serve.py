@@ -1,0 +1,78 @@
+"""Continuous-batching server: every request's result is the same alone, in
+a mix with other requests, and as a plain mcg_solve; admission respects the
+device-memory budget; poll / cancel semantics."""
+import numpy as np
+import pytest
+
+from montecarlocpp_amd import abi, engine, materials, scenes, serve
+from tests.helpers import rownorm_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def drain(sv, ids):
+    res = {}
+    for _ in range(100000):
+        sv.step()
+        for rid in ids:
+            if rid not in res:
+                r = sv.poll(rid)
+                if r is not None:
+                    res[rid] = r
+        if len(res) == len(ids):
+            return res
+    raise AssertionError("server did not finish")
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_FLIGHT])
+def test_alone_equals_mixed_equals_solve(precision):
+    fac = serve.RequestFactory(nmin=2e4, nmax=3e5, precision=precision)
+    reqs = [fac.make(r) for r in range(1, 13)]
+    target = reqs[3]
+    print("target family", target[0], "n", target[3].nemit, flush=True)
+    with engine.Server(step_bundles=50_000) as sv:
+        a = sv.submit(*target[1:])
+        alone = drain(sv, [a])[a]
+    with engine.Server(step_bundles=70_000) as sv:
+        ids = [sv.submit(*r[1:]) for r in reqs]
+        mixed = drain(sv, ids)
+    m = mixed[ids[3]]
+    with engine.Scene(target[1], target[2]) as sc:
+        f1, s1, _ = sc.solve(target[3])
+    for got, st in ((alone[0], alone[1]), (m[0], m[1])):
+        assert st["steps"] == s1["steps"] and st["bundles"] == s1["bundles"]
+        assert np.all(rownorm_rel(got, f1) < 1e-10)
+    # every other request of the mix also equals its own solve
+    for rid, r in zip(ids, reqs):
+        with engine.Scene(r[1], r[2]) as sc:
+            f, st1, _ = sc.solve(r[3])
+        assert mixed[rid][1]["steps"] == st1["steps"]
+        assert np.all(rownorm_rel(mixed[rid][0], f) < 1e-10)
+
+
+def test_budget_admission_and_cancel():
+    fac = serve.RequestFactory(nmin=1e4, nmax=1e5)
+    reqs = [fac.make(r) for r in range(1, 7)]
+    # each request holds ~125 KB of device memory: a 200 KB budget admits one
+    with engine.Server(budget_bytes=200_000, step_bundles=20_000) as sv:
+        ids = [sv.submit(*r[1:]) for r in reqs]
+        a, q, _ = sv.step()
+        assert a == 1 and q == 5
+        info = sv.info()
+        assert info["bytes_used"] <= info["bytes_budget"]
+        sv.cancel(ids[-1])
+        res = drain(sv, ids[:-1])
+        assert len(res) == 5
+        with pytest.raises(engine.EngineError) as e:
+            sv.poll(ids[-1])
+        assert e.value.code == abi.MCG_ENOTFOUND
+    with engine.Server(budget_bytes=1000) as sv:
+        with pytest.raises(engine.EngineError, match="budget"):
+            sv.submit(*reqs[0][1:])
+
+
+def test_open_loop_latency_report():
+    out = serve.run(lam=200.0, nreq=40, factory=serve.RequestFactory(nmin=1e4, nmax=1e6))
+    assert out["requests"] == 40
+    assert 0 < out["latency_p50_s"] <= out["latency_p99_s"]
+    assert out["scatter_steps_per_s"] > 0
