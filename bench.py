@@ -12,8 +12,18 @@ tallies and sampling tables.  One timed step is one complete field solve
 Printed: one JSON line with the scatter-step throughput, the roofline
 object of the trajectory kernel (algorithmic bytes = 96 B of bundle state
 round trip per scatter-step), the tally error of the FP32-flight run against
-the FP64 run of the same streams, and the FP64 CPU checker timed on a
-bounded sample of the same workload on the host cores.
+the FP64 run of the same streams, the FP64 CPU checker timed on a bounded
+sample of the same workload on the host cores, and two secondary objects:
+"batched" (256 independent slab solves in one persistent launch against the
+same solves run one mcg_solve at a time) and "serving" (an open-loop Poisson
+request stream through the continuous-batching server, with p50/p99
+latency).  --headline-only skips the secondary objects and the CPU leg.
+
+    python bench.py --copy-probe
+
+runs only a 1 GiB device-to-device copy (torch): profiling it with
+rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE gives the counters' scale for a
+known traffic, against which the trajectory kernel's counters are read.
 """
 import argparse
 import json
@@ -52,7 +62,28 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the bar workload (no CPU leg, batched or serving objects)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy for counter calibration and exit")
+    ap.add_argument("--traffic-json",
+                    default=os.path.join(ROOT, "profiles", "trajectory_kernel_bar1e7_fp32_traffic.json"),
+                    help="counter-measured HBM bytes per launch of the kernel (rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE, scaled by the copy-probe calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": x.numel() * 8, "copies": 3}))
+        return
+    if args.headline_only:
+        args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the solve shards by "
               "bundle range across GPUs, which this build does not run.", file=sys.stderr)
@@ -111,6 +142,16 @@ def main():
                          f"workload, FP64, OpenMP static schedule over bundles "
                          f"({cst['steps']} scatter-steps in {ct:.2f} s)"}
 
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        with open(args.traffic_json) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    batched = serving = None
+    if not args.headline_only:
+        batched = bench_batched()
+        serving = bench_serving()
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -138,15 +179,55 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "trajectory_kernel<float, true, true>",
             "kernel_ms_mean": mean_kern_s * 1e3,
             "bytes_per_step": BYTES_PER_STEP_FP32,
         },
         "tally_error": tally_error,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
+
+
+def bench_batched():
+    """256 independent slabs (random thickness, T_ref, lateral specularity,
+    bundle count): one persistent launch vs one mcg_solve per slab."""
+    from montecarlocpp_amd import workloads
+    cases = workloads.slab_batch(256, precision=abi.FP32_FLIGHT)
+    scs = [engine.Scene(m, d) for m, d, _ in cases]
+    probs = [p for _, _, p in cases]
+    engine.batch_solve(scs, probs)
+    t0 = time.perf_counter()
+    _, st = engine.batch_solve(scs, probs)
+    tb = time.perf_counter() - t0
+    steps = sum(x["steps"] for x in st)
+    t0 = time.perf_counter()
+    single_kernel_ms = 0.0
+    for sc, p in zip(scs, probs):
+        single_kernel_ms += sc.solve(p)[1]["kernel_ms"]
+    ts = time.perf_counter() - t0
+    for sc in scs:
+        sc.close()
+    return {"workload": "256 slabs, L log-uniform 10 nm-10 um, T_ref 100-500 K, dT 2 K, "
+                        "lateral specularity U[0,1], 1e4-1e6 bundles each (5.4e7 total), FP32 flight",
+            "bundles": int(sum(p.nemit for p in probs)), "scatter_steps": int(steps),
+            "batched_steps_per_s": steps / tb, "batched_ms": tb * 1e3,
+            "batched_kernel_ms": st[0]["kernel_ms"],
+            "single_solve_path_steps_per_s": steps / ts, "single_solve_path_ms": ts * 1e3,
+            "single_solve_path_kernel_ms": single_kernel_ms,
+            "speedup_vs_single_solve_path": ts / tb}
+
+
+def bench_serving(lam=50.0, nreq=300):
+    """Open-loop Poisson stream of slab / bar / random-slab requests
+    (1e4-1e8 bundles, T_ref 100-500 K) through the continuous-batching server."""
+    from montecarlocpp_amd import serve
+    out = serve.run(lam, nreq, factory=serve.RequestFactory())
+    out["requests_mix"] = "slab / bar / random slab, bundles log-uniform 1e4-1e8, FP32 flight"
+    return out
 
 
 if __name__ == "__main__":

@@ -269,11 +269,13 @@ int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec*
  * Continuous batching.  A request is one field solve (its own material,
  * domain and problem).  mcg_server_step admits queued requests in arrival
  * order while their device memory fits the budget, runs one persistent batch
- * launch over fair shares of every active request's remaining bundles (at
- * most step_bundles in total), and retires the requests whose bundles are all
- * done.  Tallies and counters of an admitted request stay on the GPU between
+ * launch over fair shares of every active request's remaining bundles
+ * (step_bundles in total, default 2^24, at least 2^16 per request), and
+ * retires the requests whose bundles are all done.  Tallies and counters of an admitted request stay on the GPU between
  * steps.  Results equal mcg_solve's for the same problem, alone or mixed with
- * other requests.  One host thread drives a server. */
+ * other requests.  Steps are pipelined: mcg_server_step enqueues the next
+ * step before it waits for the previous one, so a request retires one call
+ * after its last bundles were handed out.  One host thread drives a server. */
 typedef struct mcg_server mcg_server;
 
 typedef struct mcg_server_stats {
@@ -289,6 +291,8 @@ int mcg_server_submit(mcg_server* server, const mcg_material_spec* mat,
                       const mcg_domain_spec* dom, const mcg_problem_spec* prob,
                       int64_t* request_id);
 int mcg_server_step(mcg_server* server, int32_t* active, int32_t* queued, int32_t* completed);
+/* Ids of the requests the last mcg_server_step retired (n = how many). */
+int mcg_server_completed(const mcg_server* server, int64_t* ids, int64_t cap, int64_t* n);
 /* MCG_OK and the result (then the request is forgotten), MCG_EAGAIN while it
  * runs, MCG_ENOTFOUND for an unknown id.  stats.wall_ms = submit to done. */
 int mcg_server_poll(mcg_server* server, int64_t request_id, double* out, int64_t out_len,

@@ -1241,6 +1241,13 @@ size_t scene_bytes(const HostScene& h) {
 // work items of all active requests (fair shares of a per-step bundle budget)
 // into one persistent batch launch per precision.  Bundles never straddle
 // steps: an item runs its bundles to termination inside the step.
+//
+// Steps are pipelined: a call plans and enqueues step k+1 (bundle cursors
+// advance when work is handed out), then waits for step k and retires the
+// requests it finished, so the GPU runs step k+1 while the host copies
+// results, admits arrivals and plans.  The server's stream does not
+// synchronise with the null stream; device memory of retired requests is
+// freed only when the server is idle, because freeing can stall the device.
 namespace mcg {
 
 struct Request {
@@ -1248,7 +1255,7 @@ struct Request {
     std::unique_ptr<mcg_scene> scene;
     HostProblem P;
     int64_t cursor = 0;     // bundles handed out (offset from P.first)
-    int64_t done = 0;       // bundles finished
+    int64_t done = 0;       // bundles of completed steps
     size_t bytes = 0;       // device memory held while admitted
     DevBuf<double> direct, diff;
     DevBuf<unsigned long long> ctr;
@@ -1257,6 +1264,21 @@ struct Request {
     std::vector<double> result;
     mcg_stats stats{};
     std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+// One enqueued step: its launches' buffers, timing events and hand-outs.
+struct StepSlot {
+    DevBuf<KParams<float>> kp32;
+    DevBuf<KParams<double>> kp64;
+    DevBuf<WorkItem> it32, it64;
+    DevBuf<unsigned int> ictr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool live = false;
+    std::vector<std::pair<Request*, int64_t>> handed;
+    ~StepSlot() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
 };
 
 }  // namespace mcg
@@ -1269,15 +1291,18 @@ struct mcg_server {
     std::deque<mcg::Request*> queue;
     std::vector<mcg::Request*> active;
     std::map<int64_t, std::unique_ptr<mcg::Request>> all;
+    std::vector<std::unique_ptr<mcg::Request>> graveyard;   // retired, device memory not yet freed
+    std::vector<int64_t> completed;                          // ids retired by the last step call
     hipStream_t stream = nullptr;
-    mcg::DevBuf<mcg::KParams<float>> kp32;
-    mcg::DevBuf<mcg::KParams<double>> kp64;
-    mcg::DevBuf<mcg::WorkItem> it32, it64;
-    mcg::DevBuf<unsigned int> ictr;
+    mcg::StepSlot slot[2];
+    int cur = 0;                                             // slot of the step in flight
     double kernel_ms_total = 0.0;
     int64_t steps_total = 0;
     ~mcg_server() {
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
     }
 };
 
@@ -1302,13 +1327,16 @@ int admit(mcg_server* sv, Request* r) {
     return MCG_OK;
 }
 
-void release(mcg_server* sv, Request* r) {
-    r->direct.alloc(0);
-    r->diff.alloc(0);
-    r->ctr.alloc(0);
-    r->cdf.alloc(0);
-    r->scene.reset();
-    sv->used -= std::min(sv->used, r->bytes);
+// Free device memory of retired requests (called when nothing is in flight).
+void bury(mcg_server* sv) {
+    for (auto& r : sv->graveyard) {
+        r->direct.alloc(0);
+        r->diff.alloc(0);
+        r->ctr.alloc(0);
+        r->cdf.alloc(0);
+        r->scene.reset();
+    }
+    sv->graveyard.clear();
 }
 
 int retire(mcg_server* sv, Request* r) {
@@ -1327,15 +1355,16 @@ int retire(mcg_server* sv, Request* r) {
     r->t_done = std::chrono::steady_clock::now();
     r->stats.wall_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     r->state = 2;
-    release(sv, r);
+    sv->used -= std::min(sv->used, r->bytes);
     if (r->stats.bundles != r->P.count) return set_error(MCG_EDEVICE, "request finished with bundles unaccounted for");
     return MCG_OK;
 }
 
-// Launch one precision group of the step; `reqs` are the requests with work.
+// Enqueue one precision group of a step on the server's stream.
 template <typename F>
-int step_group(mcg_server* sv, const std::vector<Request*>& reqs, const std::vector<int64_t>& take,
-               DevBuf<KParams<F>>& kpbuf, DevBuf<WorkItem>& itbuf, unsigned int* ictr, float* ms) {
+int enqueue_group(mcg_server* sv, StepSlot& slot, const std::vector<Request*>& reqs,
+                  const std::vector<int64_t>& take, DevBuf<KParams<F>>& kpbuf,
+                  DevBuf<WorkItem>& itbuf, unsigned int* ictr) {
     const int n = int(reqs.size());
     std::vector<mcg_scene*> sc(n);
     std::vector<HostProblem> P(n);
@@ -1346,7 +1375,13 @@ int step_group(mcg_server* sv, const std::vector<Request*>& reqs, const std::vec
     BatchPlan B;
     int rc = plan_shape<F>(n, sc.data(), P.data(), &sv->tune, B);
     if (rc) return rc;
-    const int64_t item = sv->tune.chunk > 0 ? sv->tune.chunk : 8192;
+    // items large enough that a workgroup's slowest bundle is a small part of
+    // its work, small enough that every CU gets items
+    int64_t handed = 0;
+    for (int64_t t : take) handed += t;
+    const int64_t item = sv->tune.chunk > 0
+                             ? sv->tune.chunk
+                             : std::max<int64_t>(8192, std::min<int64_t>(1 << 17, handed / (int64_t(B.grid) * 2)));
     std::vector<WorkItem> items;
     std::vector<KParams<F>> K(n);
     for (int i = 0; i < n; ++i) {
@@ -1361,26 +1396,69 @@ int step_group(mcg_server* sv, const std::vector<Request*>& reqs, const std::vec
         for (int64_t b = 0; b < take[i]; b += item)
             items.push_back(WorkItem{i, int32_t(std::min<int64_t>(item, take[i] - b)),
                                      r->P.first + r->cursor + b});
+        r->cursor += take[i];
+        slot.handed.push_back({r, take[i]});
     }
     if (kpbuf.n < size_t(n) && (rc = kpbuf.alloc(size_t(n) * 2))) return rc;
     if (itbuf.n < items.size() && (rc = itbuf.alloc(items.size() * 2))) return rc;
-    HIP_TRY(hipMemcpyAsync(kpbuf.p, K.data(), K.size() * sizeof(KParams<F>), hipMemcpyHostToDevice, sv->stream));
-    HIP_TRY(hipMemcpyAsync(itbuf.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, sv->stream));
+    // synchronous copies: kpbuf / itbuf of this slot are not in use on the device
+    HIP_TRY(hipMemcpy(kpbuf.p, K.data(), K.size() * sizeof(KParams<F>), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(itbuf.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     HIP_TRY(hipMemsetAsync(ictr, 0, sizeof(unsigned int), sv->stream));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, sv->stream));
-    if ((rc = launch_batch<F>(B, kpbuf.p, itbuf.p, int(items.size()), ictr, sv->stream))) return rc;
-    HIP_TRY(hipEventRecord(e1, sv->stream));
-    HIP_TRY(hipEventSynchronize(e1));
-    HIP_TRY(hipEventElapsedTime(ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (int i = 0; i < n; ++i) {
-        reqs[i]->cursor += take[i];
-        reqs[i]->done += take[i];
+    return launch_batch<F>(B, kpbuf.p, itbuf.p, int(items.size()), ictr, sv->stream);
+}
+
+// Hand out fair shares of the step budget and enqueue the step in `slot`.
+int enqueue_step(mcg_server* sv, StepSlot& slot) {
+    slot.handed.clear();
+    slot.live = false;
+    std::vector<Request*> g32, g64;
+    std::vector<int64_t> t32, t64;
+    int with_work = 0;
+    for (Request* r : sv->active)
+        if (r->cursor < r->P.count) ++with_work;
+    if (!with_work) return MCG_OK;
+    const int64_t share = std::max<int64_t>(1 << 16, sv->step_bundles / with_work);
+    for (Request* r : sv->active) {
+        const int64_t take = std::min(share, r->P.count - r->cursor);
+        if (take <= 0) continue;
+        (r->P.precision == MCG_FP64 ? g64 : g32).push_back(r);
+        (r->P.precision == MCG_FP64 ? t64 : t32).push_back(take);
     }
+    int rc;
+    if (!slot.e0) HIP_TRY(hipEventCreate(&slot.e0));
+    if (!slot.e1) HIP_TRY(hipEventCreate(&slot.e1));
+    if (!slot.ictr.p && (rc = slot.ictr.alloc(2))) return rc;
+    HIP_TRY(hipEventRecord(slot.e0, sv->stream));
+    if (!g32.empty() && (rc = enqueue_group<float>(sv, slot, g32, t32, slot.kp32, slot.it32, slot.ictr.p))) return rc;
+    if (!g64.empty() && (rc = enqueue_group<double>(sv, slot, g64, t64, slot.kp64, slot.it64, slot.ictr.p + 1))) return rc;
+    HIP_TRY(hipEventRecord(slot.e1, sv->stream));
+    slot.live = true;
+    return MCG_OK;
+}
+
+// Wait for the step in `slot` and retire the requests it completed.
+int finish_step(mcg_server* sv, StepSlot& slot) {
+    if (!slot.live) return MCG_OK;
+    HIP_TRY(hipEventSynchronize(slot.e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, slot.e0, slot.e1));
+    sv->kernel_ms_total += ms;
+    slot.live = false;
+    for (auto& h : slot.handed) h.first->done += h.second;
+    slot.handed.clear();
+    std::vector<Request*> still;
+    for (Request* r : sv->active) {
+        if (r->done >= r->P.count) {
+            int rc = retire(sv, r);
+            if (rc) return rc;
+            sv->steps_total += r->stats.steps;
+            sv->completed.push_back(r->id);
+        } else {
+            still.push_back(r);
+        }
+    }
+    sv->active.swap(still);
     return MCG_OK;
 }
 
@@ -1696,11 +1774,9 @@ int mcg_server_create(uint64_t hbm_budget_bytes, int64_t step_bundles, const mcg
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return mcg::set_error(MCG_EDEVICE, "no HIP device available");
     sv->budget = hbm_budget_bytes > 0 ? size_t(hbm_budget_bytes) : size_t(200) << 30;
-    sv->step_bundles = step_bundles > 0 ? step_bundles : (int64_t(1) << 22);
+    sv->step_bundles = step_bundles > 0 ? step_bundles : (int64_t(1) << 24);
     if (tune) sv->tune = *tune;
-    HIP_TRY(hipStreamCreate(&sv->stream));
-    int rc = sv->ictr.alloc(2);
-    if (rc) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
     *out = sv.release();
     return MCG_OK;
 }
@@ -1731,56 +1807,38 @@ int mcg_server_submit(mcg_server* sv, const mcg_material_spec* mat, const mcg_do
 int mcg_server_step(mcg_server* sv, int32_t* active, int32_t* queued, int32_t* completed) {
     if (!sv) return mcg::set_error(MCG_EINVAL, "NULL server");
     int rc;
+    sv->completed.clear();
     // admission in arrival order while the budget allows
     while (!sv->queue.empty() && sv->used + sv->queue.front()->bytes <= sv->budget) {
         mcg::Request* r = sv->queue.front();
         sv->queue.pop_front();
         if ((rc = mcg::admit(sv, r))) return rc;
     }
-    // fair share of the step's bundle budget
-    std::vector<mcg::Request*> g32, g64;
-    std::vector<int64_t> t32, t64;
-    int with_work = 0;
-    for (mcg::Request* r : sv->active)
-        if (r->cursor < r->P.count) ++with_work;
-    const int64_t share = with_work ? std::max<int64_t>(1024, sv->step_bundles / with_work) : 0;
-    for (mcg::Request* r : sv->active) {
-        const int64_t take = std::min(share, r->P.count - r->cursor);
-        if (take <= 0) continue;
-        if (r->P.precision == MCG_FP64) {
-            g64.push_back(r);
-            t64.push_back(take);
-        } else {
-            g32.push_back(r);
-            t32.push_back(take);
-        }
+    // enqueue the next step behind the one in flight, then finish that one
+    const int nxt = 1 - sv->cur;
+    if ((rc = mcg::enqueue_step(sv, sv->slot[nxt]))) return rc;
+    if ((rc = mcg::finish_step(sv, sv->slot[sv->cur]))) return rc;
+    sv->cur = nxt;
+    if (!sv->slot[sv->cur].live) {
+        // nothing in flight: retire what the step just enqueued would have, and
+        // release the memory of retired requests
+        bool again = false;
+        for (mcg::Request* r : sv->active) again = again || r->cursor < r->P.count;
+        if (!again && !sv->graveyard.empty()) mcg::bury(sv);
+    } else if (sv->graveyard.size() > 4096) {
+        mcg::bury(sv);   // bound host-side bookkeeping under continuous load
     }
-    float ms = 0.f;
-    if (!g32.empty()) {
-        if ((rc = mcg::step_group<float>(sv, g32, t32, sv->kp32, sv->it32, sv->ictr.p, &ms))) return rc;
-        sv->kernel_ms_total += ms;
-    }
-    if (!g64.empty()) {
-        if ((rc = mcg::step_group<double>(sv, g64, t64, sv->kp64, sv->it64, sv->ictr.p + 1, &ms))) return rc;
-        sv->kernel_ms_total += ms;
-    }
-    // retire finished requests
-    int ndone = 0;
-    std::vector<mcg::Request*> still;
-    for (mcg::Request* r : sv->active) {
-        if (r->done >= r->P.count) {
-            if ((rc = mcg::retire(sv, r))) return rc;
-            sv->steps_total += r->stats.steps;
-            ++ndone;
-        } else {
-            still.push_back(r);
-        }
-    }
-    sv->active.swap(still);
     if (active) *active = int32_t(sv->active.size());
     if (queued) *queued = int32_t(sv->queue.size());
-    if (completed) *completed = ndone;
+    if (completed) *completed = int32_t(sv->completed.size());
     return MCG_OK;
+}
+
+int mcg_server_completed(const mcg_server* sv, int64_t* ids, int64_t cap, int64_t* n) {
+    if (!sv || !n) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    *n = int64_t(sv->completed.size());
+    for (int64_t i = 0; i < *n && i < cap; ++i) ids[i] = sv->completed[size_t(i)];
+    return *n > cap ? mcg::set_error(MCG_ERANGE, "id buffer too small") : MCG_OK;
 }
 
 int mcg_server_poll(mcg_server* sv, int64_t id, double* out, int64_t out_len, mcg_stats* stats) {
@@ -1794,6 +1852,7 @@ int mcg_server_poll(mcg_server* sv, int64_t id, double* out, int64_t out_len, mc
         std::memcpy(out, r->result.data(), r->result.size() * sizeof(double));
     }
     if (stats) *stats = r->stats;
+    sv->graveyard.push_back(std::move(it->second));
     sv->all.erase(it);
     return MCG_OK;
 }
@@ -1806,9 +1865,15 @@ int mcg_server_cancel(mcg_server* sv, int64_t id) {
     if (r->state == 0) {
         sv->queue.erase(std::remove(sv->queue.begin(), sv->queue.end(), r), sv->queue.end());
     } else if (r->state == 1) {
-        sv->active.erase(std::remove(sv->active.begin(), sv->active.end(), r), sv->active.end());
-        mcg::release(sv, r);
+        // the request may still have work in flight: wait for it first
+        int rc = mcg::finish_step(sv, sv->slot[sv->cur]);
+        if (rc) return rc;
+        if (r->state == 1) {
+            sv->active.erase(std::remove(sv->active.begin(), sv->active.end(), r), sv->active.end());
+            sv->used -= std::min(sv->used, r->bytes);
+        }
     }
+    sv->graveyard.push_back(std::move(it->second));
     sv->all.erase(it);
     return MCG_OK;
 }
@@ -1826,7 +1891,8 @@ int mcg_server_info(const mcg_server* sv, mcg_server_stats* out) {
 
 int mcg_server_destroy(mcg_server* sv) {
     if (!sv) return MCG_OK;
-    for (mcg::Request* r : sv->active) mcg::release(sv, r);
+    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+    mcg::bury(sv);
     delete sv;
     return MCG_OK;
 }

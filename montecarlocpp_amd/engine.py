@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmcg.so")
 EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_problem_plan",
            "mcg_alias_build", "mcg_philox4x32_10", "mcg_scene_create", "mcg_scene_destroy",
            "mcg_scene_plan", "mcg_scene_tune", "mcg_autotune", "mcg_autotune_batch",
-           "mcg_server_create", "mcg_server_submit", "mcg_server_step", "mcg_server_poll",
+           "mcg_server_create", "mcg_server_submit", "mcg_server_step", "mcg_server_completed",
+           "mcg_server_poll",
            "mcg_server_cancel", "mcg_server_info", "mcg_server_destroy",
            "mcg_solve", "mcg_trace", "mcg_batch_solve")
 
@@ -59,6 +60,8 @@ def lib():
         L.mcg_server_submit.argtypes = [C.c_void_p, MS, DS, PS, C.POINTER(C.c_int64)]
         L.mcg_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32)]
+        L.mcg_server_completed.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int64,
+                                           C.POINTER(C.c_int64)]
         L.mcg_server_poll.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_int64,
                                       C.POINTER(abi.Stats)]
         L.mcg_server_cancel.argtypes = [C.c_void_p, C.c_int64]
@@ -218,7 +221,7 @@ def autotune_batch(scenes_, probs, sample_fraction=0.1):
 class Server:
     """Continuous-batching request server (mcg_server_*)."""
 
-    def __init__(self, budget_bytes=200 << 30, step_bundles=1 << 22, tune=None):
+    def __init__(self, budget_bytes=200 << 30, step_bundles=1 << 24, tune=None):
         h = C.c_void_p()
         check(lib().mcg_server_create(int(budget_bytes), int(step_bundles),
                                       C.byref(tune) if tune is not None else None, C.byref(h)))
@@ -254,6 +257,13 @@ class Server:
         a, q, d = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().mcg_server_step(self.handle, C.byref(a), C.byref(q), C.byref(d)))
         return a.value, q.value, d.value
+
+    def completed(self):
+        """Ids the last step retired."""
+        ids = (C.c_int64 * 65536)()
+        n = C.c_int64()
+        check(lib().mcg_server_completed(self.handle, ids, 65536, C.byref(n)))
+        return list(ids[:n.value])
 
     def poll(self, rid):
         """(field, stats) when done, None while running."""

@@ -14,11 +14,8 @@ def drain(sv, ids):
     res = {}
     for _ in range(100000):
         sv.step()
-        for rid in ids:
-            if rid not in res:
-                r = sv.poll(rid)
-                if r is not None:
-                    res[rid] = r
+        for rid in sv.completed():
+            res[rid] = sv.poll(rid)
         if len(res) == len(ids):
             return res
     raise AssertionError("server did not finish")
