@@ -17,7 +17,8 @@ sample of the same workload on the host cores, and two secondary objects:
 "batched" (256 independent slab solves in one persistent launch against the
 same solves run one mcg_solve at a time) and "serving" (an open-loop Poisson
 request stream through the continuous-batching server, with p50/p99
-latency).  --headline-only skips the secondary objects and the CPU leg.
+latency), plus "porous_film" (the periodic porous film with a cylindrical
+pore, 1e8 bundles, 64 x 64 map, with its own CPU baseline sample).  --headline-only skips the secondary objects and the CPU leg.
 
     python bench.py --copy-probe
 
@@ -50,6 +51,11 @@ def workload():
     mat = materials.si_like(300.0)
     dom = scenes.bar(1e-6, 1e-7, 1e-7, 100, t_hot=10.0, t_cold=-10.0)
     return mat, dom
+
+
+def progress(msg):
+    """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def rel(a, b):
@@ -105,8 +111,10 @@ def main():
             kern_ms.append(st["kernel_ms"])
             fields.append(f)
         elapsed = time.perf_counter() - t0
+        progress(f"headline: {args.steps} solves in {elapsed:.2f} s")
         # precision split: the same streams in FP64
         f64, st64, _ = sc.solve(scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64))
+        progress("FP64 solve done")
 
     value = steps / elapsed
     ms_per_step = elapsed / args.steps * 1e3
@@ -141,16 +149,21 @@ def main():
                "sample": f"first {CPU_SAMPLE} of the {NBUNDLES} bundles of the same bar "
                          f"workload, FP64, OpenMP static schedule over bundles "
                          f"({cst['steps']} scatter-steps in {ct:.2f} s)"}
+        progress(f"CPU baseline done in {ct:.1f} s")
 
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
         with open(args.traffic_json) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    batched = serving = None
+    batched = serving = porous = None
     if not args.headline_only:
+        porous = bench_porous(cpu=not args.no_cpu)
+        progress("porous film done")
         batched = bench_batched()
+        progress("batched done")
         serving = bench_serving()
+        progress("serving done")
 
     line = {
         "metric": METRIC,
@@ -186,10 +199,51 @@ def main():
         },
         "tally_error": tally_error,
         "cpu_baseline": cpu,
+        "porous_film": porous,
         "batched": batched,
         "serving": serving,
     }
     print(json.dumps(line))
+
+
+def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
+    """Periodic porous film (cylindrical pore, 64 x 64 map), 1e8 bundles,
+    FP32 flight, maxscat 100: one full solve per timed step."""
+    from montecarlocpp_amd import workloads
+    mat, dom, prob = workloads.porous_film(nbundles)
+    vol = engine.cell_volumes(dom)
+    with engine.Scene(mat, dom) as sc:
+        sc.solve(workloads.porous_film(nbundles // 10)[2])
+        runs = []
+        for k in range(2):
+            p = workloads.porous_film(nbundles, seed=1 + k)[2]
+            t0 = time.perf_counter()
+            f, st, _ = sc.solve(p)
+            runs.append((time.perf_counter() - t0, f, st))
+            progress(f"porous film solve {k}: {st['steps']} steps in {runs[-1][0]:.2f} s")
+    wall = sum(r[0] for r in runs)
+    steps = sum(r[2]["steps"] for r in runs)
+    kern = sum(r[2]["kernel_ms"] for r in runs) * 1e-3
+    keff = [workloads.k_eff_porous(r[1], vol) for r in runs]
+    out = {"workload": "porous film 300x300x100 nm, centred pore r = 100 nm (diffuse), periodic "
+                       "x (dT 1 K) and y, diffuse top/bottom, Si-like at 300 K, 1e8 bundles, "
+                       "64x64 Multi map, FP32 flight, maxscat 100",
+           "bundles": nbundles, "scatter_steps_per_solve": steps / len(runs),
+           "steps_per_s": steps / wall, "kernel_steps_per_s": steps / kern,
+           "ms_per_solve": wall / len(runs) * 1e3,
+           "k_eff_W_per_mK": float(np.mean(keff)), "k_eff_seed_spread": float(abs(keff[0] - keff[1])),
+           "roofline_frac_96B": steps / kern * BYTES_PER_STEP_FP32 / HBM_PEAK}
+    if cpu:
+        from oracle import oracle  # CPU baseline leg only
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        cp = workloads.porous_film(cpu_sample, precision=abi.FP64)[2]
+        t0 = time.perf_counter()
+        _, cst, _ = oracle.solve(mat, dom, cp, threads=threads)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{cpu_sample} bundles of the same porous film, FP64"}
+    return out
 
 
 def bench_batched():
@@ -222,11 +276,12 @@ def bench_batched():
 
 
 def bench_serving(lam=50.0, nreq=300):
-    """Open-loop Poisson stream of slab / bar / random-slab requests
+    """Open-loop Poisson stream of slab / bar / random-slab / porous-film requests
     (1e4-1e8 bundles, T_ref 100-500 K) through the continuous-batching server."""
     from montecarlocpp_amd import serve
     out = serve.run(lam, nreq, factory=serve.RequestFactory())
-    out["requests_mix"] = "slab / bar / random slab, bundles log-uniform 1e4-1e8, FP32 flight"
+    out["requests_mix"] = ("slab / bar / random slab / porous film, bundles log-uniform 1e4-1e8, "
+                           "FP32 flight")
     return out
 
 

@@ -214,6 +214,12 @@ int mcg_material_compute(const mcg_material_spec* mat, mcg_material_tables* out)
 int mcg_problem_plan(const mcg_material_spec* mat, const mcg_domain_spec* dom,
                      const mcg_problem_spec* prob, mcg_problem_info* info,
                      double* emit_weight, int64_t* emit_count, int64_t nemitter_cap);
+/* Per-column tally cell volumes of a domain (the CellVolF of problem.cpp:302-306
+ * over Field's columns, field.cpp:53-77): out receives min(cap, ncols)
+ * doubles, *ncols the column count.  A cell a pore cuts reports its material
+ * volume; a cell wholly inside a pore reports 0 (its normalised tallies are
+ * 0). */
+int mcg_cell_volumes(const mcg_domain_spec* dom, double* out, int64_t cap, int64_t* ncols);
 int mcg_alias_build(const double* pdf, int64_t n, uint32_t* thresh, uint32_t* alias);
 void mcg_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

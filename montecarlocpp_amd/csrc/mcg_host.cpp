@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "mcg_disk.h"
 #include "mcg_scene.h"
 
 namespace mcg {
@@ -337,6 +338,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
     s.emit_weight.clear();
     s.inter_list.clear();
     s.pores.clear();
+    s.has_1d = s.has_grid = false;
     const int ns = spec->nsdom;
     std::vector<double> face_temp;            // pre-pairing temperature per plane
     std::vector<double> face_area;
@@ -381,6 +383,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             s.has_1d = true;
         } else {
             d.accum = positive + 1;
+            s.has_grid = true;
         }
         for (int k = 0; k < 3; ++k) {
             if (shapes[si][k] > (1ll << 30)) return set_error(MCG_ERANGE, "too many tally divisions");
@@ -506,6 +509,16 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
         if (ps.kind != MCG_DIFF && ps.kind != MCG_SPEC && ps.kind != MCG_PARTIAL)
             return set_error(MCG_EINVAL, "pore wall must be Diff, Spec or Partial");
         if (s.sdoms[ps.sdom].pore >= 0) return set_error(MCG_EINVAL, "one pore per subdomain");
+        const mcg_subdomain_spec& c = spec->sdom[ps.sdom];
+        const double* m = c.mat;
+        if (c.cell != MCG_PARALLELEPIPED || m[1] != 0.0 || m[2] != 0.0 || m[3] != 0.0 || m[5] != 0.0 ||
+            m[6] != 0.0 || m[7] != 0.0)
+            return set_error(MCG_EINVAL, "a pore needs an axis-aligned box cell");
+        if (ps.center[0] - ps.radius < c.origin[0] || ps.center[0] + ps.radius > c.origin[0] + m[0] ||
+            ps.center[1] - ps.radius < c.origin[1] || ps.center[1] + ps.radius > c.origin[1] + m[4])
+            return set_error(MCG_EINVAL, "pore must lie inside the cell");
+        if (ps.kind == MCG_PARTIAL && !(ps.specularity >= 0.0 && ps.specularity <= 1.0))
+            return set_error(MCG_EINVAL, "specularity outside [0, 1]");
         DevPore<double> P{};
         P.cx = ps.center[0];
         P.cy = ps.center[1];
@@ -523,10 +536,12 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
     for (int si = 0; si < ns; ++si) {
         const mcg_subdomain_spec& c = spec->sdom[si];
         const Vec g = mk(c.grad_t[0], c.grad_t[1], c.grad_t[2]);
-        const double wv = 2.0 * sdom_vol[si] * len(g);
+        // a pore removes its cylinder from the volume that emits
+        const int pk = s.sdoms[si].pore;
+        const double vmat = pk < 0 ? sdom_vol[si]
+                                   : sdom_vol[si] - 3.141592653589793 * (s.pores[pk].r * s.pores[pk].r) * c.mat[8];
+        const double wv = 2.0 * vmat * len(g);
         if (wv != 0.0) {
-            if (c.cell != MCG_PARALLELEPIPED && spec->npore > 0)
-                return set_error(MCG_EINVAL, "volume emission with pores needs a box cell");
             DevEmitter<double> e{};
             for (int k = 0; k < 3; ++k) {
                 e.o[k] = c.origin[k];
@@ -578,7 +593,19 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             for (int64_t j = 0; j < sh[1]; ++j)
                 for (int64_t i = 0; i < sh[0]; ++i) {
                     const int64_t idx[3] = {i, j, k};
-                    s.cell_vol.push_back(volume_of_cell(spec->sdom[si].cell, sdom_vol[si], sh, idx));
+                    double v = volume_of_cell(spec->sdom[si].cell, sdom_vol[si], sh, idx);
+                    if (d.pore >= 0) {   // material part of a cell the pore cuts
+                        const DevPore<double>& P = s.pores[d.pore];
+                        const mcg_subdomain_spec& c = spec->sdom[si];
+                        const double lx = c.mat[0], ly = c.mat[4];
+                        const double x0 = c.origin[0] + lx * (double(i) / double(sh[0]));
+                        const double x1 = c.origin[0] + lx * (double(i + 1) / double(sh[0]));
+                        const double y0 = c.origin[1] + ly * (double(j) / double(sh[1]));
+                        const double y1 = c.origin[1] + ly * (double(j + 1) / double(sh[1]));
+                        const double in = mcg_disk::rect_in_disk(x0, x1, y0, y1, P.cx, P.cy, P.r);
+                        v = v * (1.0 - in / ((x1 - x0) * (y1 - y0)));
+                    }
+                    s.cell_vol.push_back(v);
                 }
         s.cols += prod;
         if (s.cols > (1ll << 30)) return set_error(MCG_ERANGE, "too many tally columns");
@@ -676,7 +703,9 @@ void finalise_field(const HostScene& s, const HostProblem& p, const double* dire
             break;
     }
     for (int64_t c = 0; c < Cn; ++c)
-        for (int64_t r = 0; r < R; ++r) out[c * R + r] = p.power * (sol[c * R + r] / s.cell_vol[c]);
+        // a cell with no material (wholly inside a pore) reports zero
+        for (int64_t r = 0; r < R; ++r)
+            out[c * R + r] = s.cell_vol[c] > 0.0 ? p.power * (sol[c * R + r] / s.cell_vol[c]) : 0.0;
 }
 
 }  // namespace mcg

@@ -35,6 +35,12 @@ constexpr int kWave = 64;
 constexpr int kChunk = 256;            // bundle ids claimed per wavefront grab
 constexpr int kMaxRows = 4;            // nonzero amount entries per step
 constexpr size_t kLdsTallyCap = 48 * 1024;
+// Emission inside a pored box: position redraw k uses Philox blocks
+// kPosRedrawBlock + 2k (two coordinates) and + 2k + 1 (the third); a pore
+// covers at most pi/4 of the cross-section, so the cap is never reached in
+// practice (probability below 1e-400).
+constexpr uint32_t kPosRedrawBlock = 64;
+constexpr uint32_t kMaxPosRedraw = 4096;
 
 enum Counter : int { C_NEXT = 0, C_STEPS, C_ESC, C_ABS, C_TRUNC, C_BUNDLES, C_COUNT };
 
@@ -64,6 +70,7 @@ struct KParams {
     uint32_t* c_nstep;
     uint32_t* c_fate;
     int32_t lds_tally;
+    int32_t lds_diff;      // the LDS tally holds the range-add difference array (1-D tallies)
     int32_t chunk;         // bundle ids a wavefront claims at a time
     int64_t trace_id;      // bundle whose steps are recorded, -1 for none
     double* trace;         // MCG_TRACE_WIDTH doubles per step
@@ -172,6 +179,19 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
         }
         const dev::V3<F> m = dev::matvec(E.mat, dev::V3<F>{c0, c1, c2});
         L.pos = dev::V3<F>{E.o[0] + m.x, E.o[1] + m.y, E.o[2] + m.z};
+        const int pk = K.npore > 0 ? K.sdoms[E.sdom].pore : -1;
+        if (pk >= 0) {   // uniform in the box minus the pore: redraw inside it
+            const DevPore<F>& Q = K.pores[pk];
+            for (uint32_t k = 0; k < kMaxPosRedraw; ++k) {
+                const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
+                if (!(dx * dx + dy * dy < Q.r2)) break;
+                const uint4 ba = ev.block(kPosRedrawBlock + 2u * k), bb = ev.block(kPosRedrawBlock + 2u * k + 1u);
+                const dev::V3<F> q = dev::matvec(
+                    E.mat, dev::V3<F>{dev::Real<F>::uniform(ba.x, ba.y), dev::Real<F>::uniform(ba.z, ba.w),
+                                      dev::Real<F>::uniform(bb.x, bb.y)});
+                L.pos = dev::V3<F>{E.o[0] + q.x, E.o[1] + q.y, E.o[2] + q.z};
+            }
+        }
         d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, true));
         L.sign = (d.x * E.grad[0] + d.y * E.grad[1] + d.z * E.grad[2] < F(0)) ? 1 : -1;
     }
@@ -270,6 +290,11 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
         }
     }
     int idx[3] = {bi[0], bi[1], bi[2]};
+    // parameter of each live axis's next crossing, recomputed only when that
+    // axis advances (same expression, so the same values as the map keys)
+    F pk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pk[k] = live[k] ? (F(nxt[k]) - bc[k]) / dc[k] : F(2);
     F prev = F(0);
     for (;;) {
         F pmin = F(2);
@@ -277,8 +302,7 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (!live[k]) continue;
-            const F p = (F(nxt[k]) - bc[k]) / dc[k];
-            if (!any || p < pmin) pmin = p;
+            if (!any || pk[k] < pmin) pmin = pk[k];
             any = true;
         }
         if (!any) break;
@@ -290,13 +314,11 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
         prev = pmin;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            if (!live[k]) continue;
-            const F p = (F(nxt[k]) - bc[k]) / dc[k];
-            if (p == pmin) {
-                idx[k] += pm[k];
-                if (nxt[k] == last[k]) live[k] = false;
-                nxt[k] += pm[k];
-            }
+            if (!live[k] || pk[k] != pmin) continue;
+            idx[k] += pm[k];
+            if (nxt[k] == last[k]) live[k] = false;
+            nxt[k] += pm[k];
+            if (live[k]) pk[k] = (F(nxt[k]) - bc[k]) / dc[k];
         }
     }
     if (prev != F(1)) {
@@ -360,7 +382,8 @@ struct View {
 
 // Stage one solve's tables, planes and subdomains in LDS and clear the LDS
 // tally; the caller synchronises.  Layout: [scat alias | vel | mfp] (when
-// kLdsTables), planes, subdomains, tally direct, tally diff.
+// kLdsTables), planes, subdomains, tally direct, tally diff (only when the
+// scene has 1-D tallies, the only users of range adds).
 template <typename F, bool kLdsTables, bool kLdsTally>
 __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int nthreads) {
     const int ntab = kLdsTables ? K.nmodes : 0;
@@ -398,10 +421,10 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
     const int ns = int(sizeof(DevSdom<F>) / 4) * K.nsdom;
     for (int i = tid; i < ns; i += nthreads) dst[i] = src[i];
     if (kLdsTally) {
-        const int n = K.rows * K.cols + K.rows * (K.cols + 1);
+        const int n = K.rows * K.cols + (K.lds_diff ? K.rows * (K.cols + 1) : 0);
         for (int i = tid; i < n; i += nthreads) s_direct[i] = 0.0;
         v.direct = s_direct;
-        v.diff = s_diff;
+        v.diff = K.lds_diff ? s_diff : K.diff;
     } else {
         v.direct = K.direct;
         v.diff = K.diff;
@@ -413,13 +436,13 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
 
 // Add a workgroup's LDS tally into the solve's global tally (after a barrier).
 __device__ void flush_tally(const double* s_direct, const double* s_diff, double* g_direct,
-                            double* g_diff, int rows, int cols, int tid, int nthreads) {
+                            double* g_diff, int rows, int cols, bool with_diff, int tid, int nthreads) {
     const int nd = rows * cols;
     for (int i = tid; i < nd; i += nthreads) {
         const double v = s_direct[i];
         if (v != 0.0) unsafeAtomicAdd(g_direct + i, v);
     }
-    const int nf = rows * (cols + 1);
+    const int nf = with_diff ? rows * (cols + 1) : 0;
     for (int i = tid; i < nf; i += nthreads) {
         const double v = s_diff[i];
         if (v != 0.0) unsafeAtomicAdd(g_diff + i, v);
@@ -667,7 +690,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     st.publish(K.counters, lane);
     if (kLdsTally) {
         __syncthreads();
-        flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, tid, blockDim.x);
+        flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, K.lds_diff != 0, tid, blockDim.x);
     }
 }
 
@@ -707,7 +730,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
         if (w.solve != cur) {
             if (cur >= 0 && kLdsTally)
                 flush_tally(V.direct, V.diff, solves[cur].direct, solves[cur].diff, solves[cur].rows,
-                            solves[cur].cols, tid, blockDim.x);
+                            solves[cur].cols, solves[cur].lds_diff != 0, tid, blockDim.x);
             __syncthreads();
             V = stage<F, kLdsTables, kLdsTally>(solves[w.solve], lds, tid, blockDim.x);
             cur = w.solve;
@@ -742,7 +765,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
     }
     if (cur >= 0 && kLdsTally)
         flush_tally(V.direct, V.diff, solves[cur].direct, solves[cur].diff, solves[cur].rows,
-                    solves[cur].cols, tid, blockDim.x);
+                    solves[cur].cols, solves[cur].lds_diff != 0, tid, blockDim.x);
 }
 
 // ------------------------------------------------------------------ host
@@ -873,8 +896,29 @@ size_t lds_bytes(const HostScene& h, int rows, bool tables, bool tally) {
     size_t off = ((tables ? h.mode_cell.size() : 0) * (8 + 2 * sizeof(F)) + 15) & ~size_t(15);
     off = (off + sizeof(DevPlane<F>) * h.planes.size() + 15) & ~size_t(15);
     off = (off + sizeof(DevSdom<F>) * h.sdoms.size() + 15) & ~size_t(15);
-    if (tally) off += sizeof(double) * (size_t(rows) * h.cols + size_t(rows) * (h.cols + 1));
+    if (tally) off += sizeof(double) * (size_t(rows) * h.cols + (h.has_1d ? size_t(rows) * (h.cols + 1) : 0));
     return off;
+}
+
+// What a launch stages in LDS.  The tables and the tally both go in when
+// they fit; a 1-D tally only while it is small (it takes O(1) adds per step
+// through range adds, so LDS spent on it costs occupancy for little), a 2-D
+// / 3-D map whatever its size (a step adds to every cell it crosses, and
+// global FP64 atomics for those bound the whole solve), at the price of the
+// tables, which then stay in global memory behind the L2.
+struct LdsPlan {
+    bool tables = false, tally = false;
+    size_t bytes = 0;
+};
+
+LdsPlan plan_lds(size_t geo, size_t tab, size_t tally, bool grid, bool tables_allowed, size_t limit) {
+    const bool tab_fits = tables_allowed && geo + tab <= limit;
+    const bool tally_ok = grid || tally <= kLdsTallyCap;
+    if (tab_fits && tally_ok && geo + tab + tally <= limit) return {true, true, geo + tab + tally};
+    if (grid && geo + tally <= limit) return {false, true, geo + tally};
+    if (tab_fits) return {true, false, geo + tab};
+    if (tally_ok && geo + tally <= limit) return {false, true, geo + tally};
+    return {false, false, geo};
 }
 
 template <typename F>
@@ -913,6 +957,7 @@ void fill_params(mcg_scene* sc, const HostProblem& p, KParams<F>& K) {
     K.first = p.first;
     K.count = p.count;
     K.chunk = sc->tune.chunk > 0 ? sc->tune.chunk : kChunk;
+    K.lds_diff = h.has_1d ? 1 : 0;
     K.trace_id = -1;
 }
 
@@ -961,12 +1006,12 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
         K.trace = tbuf.p;
         K.trace_cap = trace_cap;
     }
-    // LDS plan: tables first, then the tally if it is small enough
-    const bool tables = sc->tune.lds_tables >= 0 && lds_bytes<F>(h, rows, true, false) <= sc->lds_limit;
-    const size_t base = lds_bytes<F>(h, rows, tables, false);
-    const size_t with_tally = lds_bytes<F>(h, rows, tables, true);
-    const bool tally = with_tally - base <= kLdsTallyCap && with_tally <= sc->lds_limit;
-    const size_t lds = tally ? with_tally : base;
+    const size_t geo = lds_bytes<F>(h, rows, false, false);
+    const LdsPlan lp = plan_lds(geo, lds_bytes<F>(h, rows, true, false) - geo,
+                                lds_bytes<F>(h, rows, false, true) - geo, h.has_grid,
+                                sc->tune.lds_tables >= 0, sc->lds_limit);
+    const bool tables = lp.tables, tally = lp.tally;
+    const size_t lds = lp.bytes;
     if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     K.lds_tally = tally ? 1 : 0;
     const void* fn = tables ? (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, true, true>)
@@ -974,8 +1019,9 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
                             : (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, false, true>)
                                      : reinterpret_cast<const void*>(trajectory_kernel<F, false, false>));
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    const int block = sc->tune.block > 0 ? sc->tune.block : 512;
     int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
+    // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
+    const int block = sc->tune.block > 0 ? sc->tune.block : (blocks_per_cu == 1 ? 1024 : 512);
     blocks_per_cu = std::min(blocks_per_cu, std::max(1, 2048 / block));
     if (sc->tune.blocks_per_cu > 0) blocks_per_cu = std::min(blocks_per_cu, int(sc->tune.blocks_per_cu));
     const int64_t needed = (p.count + block - 1) / block;
@@ -1043,26 +1089,28 @@ template <typename F>
 int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_tune* tune,
                BatchPlan& B) {
     const mcg_scene* s0 = scenes[0];
-    size_t lds_tables = 0, lds_plain = 0, tally_bytes = 0;
+    size_t geo = 0, tab = 0, tally_bytes = 0;
+    bool grid = false;
     for (int i = 0; i < n; ++i) {
         const HostScene& h = scenes[i]->host;
         if (scenes[i]->device != s0->device) return set_error(MCG_EINVAL, "batch scenes on different devices");
-        lds_tables = std::max(lds_tables, lds_bytes<F>(h, int(P[i].rows), true, false));
-        lds_plain = std::max(lds_plain, lds_bytes<F>(h, int(P[i].rows), false, false));
-        tally_bytes = std::max(tally_bytes, lds_bytes<F>(h, int(P[i].rows), false, true) -
-                                                lds_bytes<F>(h, int(P[i].rows), false, false));
+        const size_t g = lds_bytes<F>(h, int(P[i].rows), false, false);
+        geo = std::max(geo, g);
+        tab = std::max(tab, lds_bytes<F>(h, int(P[i].rows), true, false) - g);
+        tally_bytes = std::max(tally_bytes, lds_bytes<F>(h, int(P[i].rows), false, true) - g);
+        grid = grid || h.has_grid;
     }
-    B.block = (tune && tune->block) ? tune->block : 512;
-    if (B.block != 64 && B.block != 128 && B.block != 256 && B.block != 512 && B.block != 1024)
-        return set_error(MCG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
-    B.tables = !(tune && tune->lds_tables < 0) && lds_tables <= s0->lds_limit;
-    const size_t base = B.tables ? lds_tables : lds_plain;
-    B.tally = tally_bytes <= kLdsTallyCap && base + tally_bytes <= s0->lds_limit;
-    B.lds = base + (B.tally ? tally_bytes : 0);
+    const LdsPlan lp = plan_lds(geo, tab, tally_bytes, grid, !(tune && tune->lds_tables < 0), s0->lds_limit);
+    B.tables = lp.tables;
+    B.tally = lp.tally;
+    B.lds = lp.bytes;
     if (B.lds > s0->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     // occupancy: LDS per CU (the per-block limit is the CU's LDS on gfx950)
     // and at most 2048 threads per CU
     int per_cu = std::max<int>(1, int(s0->lds_limit / std::max<size_t>(B.lds, 1)));
+    B.block = (tune && tune->block) ? tune->block : (per_cu == 1 ? 1024 : 512);
+    if (B.block != 64 && B.block != 128 && B.block != 256 && B.block != 512 && B.block != 1024)
+        return set_error(MCG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
     per_cu = std::min(per_cu, std::max(1, 2048 / B.block));
     if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, int(tune->blocks_per_cu));
     B.grid = s0->num_cu * per_cu;
@@ -1543,6 +1591,16 @@ int mcg_problem_plan(const mcg_material_spec* mat, const mcg_domain_spec* dom,
         if (emit_weight) emit_weight[i] = s.emit_weight[i];
         if (emit_count) emit_count[i] = p.emit_count[i];
     }
+    return MCG_OK;
+}
+
+int mcg_cell_volumes(const mcg_domain_spec* dom, double* out, int64_t cap, int64_t* ncols) {
+    if (!ncols || (cap > 0 && !out)) return mcg::set_error(MCG_EINVAL, "NULL output");
+    mcg::HostScene s;
+    int rc = mcg::build_domain(dom, s);
+    if (rc) return rc;
+    *ncols = s.cols;
+    for (int64_t i = 0; i < s.cols && i < cap; ++i) out[i] = s.cell_vol[i];
     return MCG_OK;
 }
 

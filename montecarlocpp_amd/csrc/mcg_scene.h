@@ -103,7 +103,8 @@ struct HostScene {
     std::vector<int32_t> inter_list;      // candidate subdomains of Inter faces
     std::vector<double> cell_vol;         // per tally column
     int64_t cols = 0;
-    bool has_1d = false;
+    bool has_1d = false;     // some subdomain tallies along one axis (range adds)
+    bool has_grid = false;   // some subdomain tallies on a 2-D / 3-D grid
 };
 
 struct HostProblem {

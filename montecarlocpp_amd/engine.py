@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmcg.so")
 
 # every function include/mcg.h declares
 EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_problem_plan",
+           "mcg_cell_volumes",
            "mcg_alias_build", "mcg_philox4x32_10", "mcg_scene_create", "mcg_scene_destroy",
            "mcg_scene_plan", "mcg_scene_tune", "mcg_autotune", "mcg_autotune_batch",
            "mcg_server_create", "mcg_server_submit", "mcg_server_step", "mcg_server_completed",
@@ -47,6 +48,7 @@ def lib():
         L.mcg_material_compute.argtypes = [MS, C.POINTER(abi.MaterialTables)]
         L.mcg_problem_plan.argtypes = [MS, DS, PS, C.POINTER(abi.ProblemInfo),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int64]
+        L.mcg_cell_volumes.argtypes = [DS, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]
         L.mcg_alias_build.argtypes = [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint32)]
         L.mcg_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
@@ -96,6 +98,15 @@ def plan(mat, dom, prob):
                                  n.ctypes.data_as(C.POINTER(C.c_int64)), 256))
     k = info.nemitter
     return info, w[:k].copy(), n[:k].copy()
+
+
+def cell_volumes(dom):
+    """Per-column tally cell volumes (host only)."""
+    n = C.c_int64()
+    check(lib().mcg_cell_volumes(C.byref(dom.spec()), None, 0, C.byref(n)))
+    out = np.zeros(n.value)
+    check(lib().mcg_cell_volumes(C.byref(dom.spec()), abi.dptr(out), out.size, C.byref(n)))
+    return out
 
 
 def material_tables(mat):

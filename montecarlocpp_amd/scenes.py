@@ -133,6 +133,24 @@ def film(dim, div, dT):
     return d
 
 
+def porous_film(dim=(300e-9, 300e-9, 100e-9), radius=100e-9, div=(64, 64, 0), dT=1.0,
+                pore=abi.DIFF, walls=abi.DIFF, specularity=0.0):
+    """Periodic porous film: a unit cell periodic in x (with the volumetric
+    gradient source -dT/dim[0], as FilmDomain, domain.cpp:188-199) and in y,
+    walls of kind ``walls`` at z = 0 and z = dim[2], and one cylindrical pore
+    along z centred in the x-y cross-section (an extension: the reference's
+    boundaries are planes)."""
+    d = Domain("porous_film")
+    wf = face(walls, 0.0, specularity)
+    s = d.add_box((0, 0, 0), dim, div,
+                  [face(abi.PERI), face(abi.PERI), wf, face(abi.PERI), face(abi.PERI), wf],
+                  grad_t=(-dT / dim[0], 0.0, 0.0))
+    d.pair(s, BAC, s, FRO, transl=(dim[0], 0.0, 0.0))
+    d.pair(s, LEF, s, RIG, transl=(0.0, dim[1], 0.0))
+    d.add_pore(s, (0.5 * dim[0], 0.5 * dim[1]), radius, kind=pore, specularity=specularity)
+    return d
+
+
 def problem(nemit, tally=abi.MULTI, maxscat=10 ** 9, maxloop=0, seed=1, precision=abi.FP64,
             size=0, first=0, count=0):
     p = abi.ProblemSpec()

@@ -2,9 +2,10 @@
 
 Requests arrive as a Poisson stream of rate lam (solves/s).  Request id r
 draws, from the Philox block (counter (r, 0, 0, 0), key (0x5EED0004, 0)) as
-uniforms u0..u3: its family (the slab, the bar, or a random slab of the
-batched workload), its bundle count N = 10^(log10 nmin + (log10 nmax -
-log10 nmin) u1) and its T_ref = 100 + 400 u2 K; its seed is r.  Requests are
+uniforms u0..u3: its family (u0), its bundle count N = 10^(log10 nmin + (log10 nmax -
+log10 nmin) u1) and its T_ref = 100 + 400 u2 K; its seed is r.  The
+families are the slab, the bar, a random slab of the batched workload and the
+periodic porous film.  Requests are
 submitted to an mcg_server as their arrival time passes and the server is
 stepped continuously; a request's latency runs from its scheduled arrival to
 the step that finished it.
@@ -21,7 +22,7 @@ import numpy as np
 from . import abi, engine, materials, scenes, workloads
 
 REQUEST_KEY = (0x5EED0004, 0)
-FAMILIES = ("slab", "bar", "random_slab")
+FAMILIES = ("slab", "bar", "random_slab", "porous_film")
 
 
 class RequestFactory:
@@ -57,6 +58,11 @@ class RequestFactory:
             bins = int(round(20 + 180 * math.log10(L / 1e-8) / 3))
             dom = scenes.slab(L, bins, side=abi.PARTIAL, specularity=u[3])
             del prm
+        if fam == "porous_film":
+            dom = scenes.porous_film(workloads.PORE_DIM, workloads.PORE_RADIUS, (64, 64, 0))
+            prob = scenes.problem(n, seed=rid, precision=self.precision,
+                                  maxscat=workloads.PORE_MAXSCAT)
+            return fam, self.material(T), dom, prob
         prob = scenes.problem(n, seed=rid, precision=self.precision)
         return fam, self.material(T), dom, prob
 

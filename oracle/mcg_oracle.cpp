@@ -42,6 +42,9 @@
 // place of libm so that CPU and GPU evaluate the same operations (see the
 // header's comment); tests/test_math.py pins them against glibc.
 #include "../montecarlocpp_amd/csrc/mcg_math.h"
+// Rectangle-disk overlap for the cell volumes of pore-cut cells (pinned by
+// tests/test_geometry.py against brute force).
+#include "../montecarlocpp_amd/csrc/mcg_disk.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -170,8 +173,9 @@ inline M3 refl_matrix(V3 n) {
 // Rotation taking the unit z axis onto n: the half-angle quaternion
 // construction (boundary.cpp:33-37 uses Eigen's Quaternion::FromTwoVectors).
 // For n antiparallel to z the rotation axis is taken to be x.
-inline M3 rot_matrix(V3 n_in) {
-    V3 n = normalized(n_in);
+inline M3 rot_unit(V3 n);
+inline M3 rot_matrix(V3 n_in) { return rot_unit(normalized(n_in)); }
+inline M3 rot_unit(V3 n) {
     double c = n.z;
     double qw, qx, qy, qz;
     if (c < -1.0 + 1e-12) {
@@ -248,8 +252,12 @@ enum : int {
     W_POS1 = 8,     // emission position draws
     W_POS2 = 10,
     W_POS3 = 12,
-    W_REDRAW = 16   // free path redraws, two words each
+    W_REDRAW = 16,  // free path redraws, two words each
+    B_POSRE = 64    // emission position redraw k: blocks 64 + 2k and 65 + 2k
 };
+// Cap on emission position redraws (a pore covers at most pi/4 of its cell's
+// cross-section, so the cap is reached with probability below 1e-400).
+constexpr int kMaxPosRedraw = 4096;
 
 struct EventRng {
     uint32_t k0, k1, s, nlo, nhi;
@@ -466,6 +474,7 @@ struct Plane {
 };
 
 struct Sdom {
+    int pore = -1;
     int cell;
     V3 o;
     M3 mat, inv;
@@ -487,7 +496,15 @@ struct Emitter {
     double weight;
 };
 
+// Extension: a cylindrical pore (axis along z) cut out of a box cell.
+struct Pore {
+    int sdom, kind;
+    double cx, cy, r, r2, eps;
+    uint32_t thr;
+};
+
 struct Domain {
+    std::vector<Pore> pores;
     std::vector<Plane> planes;
     std::vector<Sdom> sdoms;
     std::vector<Emitter> emitters;
@@ -569,7 +586,7 @@ double cell_volume(const Sdom& s, const int64_t idx[3]) {
 
 int build_domain(const mcg_domain_spec* d, Domain& D) {
     if (!d || d->nsdom <= 0 || !d->sdom) return fail(MCG_EINVAL, "Domain setup not complete");
-    if (d->npore > 0) return fail(MCG_EINVAL, "the oracle does not model pores");
+    D.pores.clear();
     D.planes.clear();
     D.sdoms.clear();
     D.emitters.clear();
@@ -714,10 +731,41 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
         if (P.kind == MCG_INTER && P.pairs.empty())
             return fail(MCG_EINVAL, "Domain setup not complete");
     }
-    // emitter registry (domain.cpp:88-102, subdomain.cpp:205-212)
+    // pores: axis-aligned box cells only, disk inside the cell's x-y extent
+    for (int k = 0; k < d->npore; ++k) {
+        const mcg_pore_spec& ps = d->pore[k];
+        if (ps.sdom < 0 || ps.sdom >= d->nsdom) return fail(MCG_EINVAL, "pore names an unknown subdomain");
+        Sdom& S = D.sdoms[ps.sdom];
+        if (S.pore >= 0) return fail(MCG_EINVAL, "one pore per subdomain");
+        const double* m = S.mat.m;
+        if (S.cell != MCG_PARALLELEPIPED || m[1] != 0.0 || m[2] != 0.0 || m[3] != 0.0 || m[5] != 0.0 ||
+            m[6] != 0.0 || m[7] != 0.0)
+            return fail(MCG_EINVAL, "a pore needs an axis-aligned box cell");
+        if (!(ps.radius > 0.0)) return fail(MCG_EINVAL, "pore radius must be positive");
+        if (ps.center[0] - ps.radius < S.o.x || ps.center[0] + ps.radius > S.o.x + m[0] ||
+            ps.center[1] - ps.radius < S.o.y || ps.center[1] + ps.radius > S.o.y + m[4])
+            return fail(MCG_EINVAL, "pore must lie inside the cell");
+        if (ps.kind != MCG_DIFF && ps.kind != MCG_SPEC && ps.kind != MCG_PARTIAL)
+            return fail(MCG_EINVAL, "pore wall must be Diff, Spec or Partial");
+        Pore P{ps.sdom, ps.kind, ps.center[0], ps.center[1], ps.radius, ps.radius * ps.radius, S.eps, 0u};
+        if (ps.kind == MCG_PARTIAL) {
+            if (!(ps.specularity >= 0.0 && ps.specularity <= 1.0))
+                return fail(MCG_EINVAL, "specularity outside [0, 1]");
+            P.thr = ps.specularity >= 1.0 ? 0xFFFFFFFFu : uint32_t(ps.specularity * 4294967296.0);
+        }
+        S.pore = int(D.pores.size());
+        D.pores.push_back(P);
+    }
+    // emitter registry (domain.cpp:88-102, subdomain.cpp:205-212); a pore
+    // removes its volume from the gradient source
     for (int si = 0; si < d->nsdom; ++si) {
         Sdom& S = D.sdoms[si];
-        double wv = 2.0 * S.vol * norm(S.gradT);
+        double vmat = S.vol;
+        if (S.pore >= 0) {
+            const Pore& P = D.pores[S.pore];
+            vmat = S.vol - 3.141592653589793 * (P.r * P.r) * S.mat.m[8];
+        }
+        double wv = 2.0 * vmat * norm(S.gradT);
         if (wv != 0.0) D.emitters.push_back({true, si, wv});
         for (int pi : faceEmit[si]) {
             const Plane& P = D.planes[pi];
@@ -739,7 +787,18 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
             for (int64_t j = 0; j < S.shape[1]; ++j)
                 for (int64_t i = 0; i < S.shape[0]; ++i) {
                     int64_t idx[3] = {i, j, k};
-                    D.cellVol.push_back(cell_volume(S, idx));
+                    double v = cell_volume(S, idx);
+                    if (S.pore >= 0) {
+                        const Pore& P = D.pores[S.pore];
+                        const double lx = S.mat.m[0], ly = S.mat.m[4];
+                        const double x0 = S.o.x + lx * (double(i) / double(S.shape[0]));
+                        const double x1 = S.o.x + lx * (double(i + 1) / double(S.shape[0]));
+                        const double y0 = S.o.y + ly * (double(j) / double(S.shape[1]));
+                        const double y1 = S.o.y + ly * (double(j + 1) / double(S.shape[1]));
+                        const double in = mcg_disk::rect_in_disk(x0, x1, y0, y1, P.cx, P.cy, P.r);
+                        v = v * (1.0 - in / ((x1 - x0) * (y1 - y0)));
+                    }
+                    D.cellVol.push_back(v);
                 }
         D.cols += prod;
     }
@@ -891,6 +950,18 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
             }
         }
         pos = add(S.o, mul(S.mat, v3(c[0], c[1], c[2])));
+        if (S.pore >= 0) {   // uniform in the box minus the pore: redraw inside it
+            const Pore& P = D.pores[S.pore];
+            for (int k = 0; k < kMaxPosRedraw; ++k) {
+                const double dx = pos.x - P.cx, dy = pos.y - P.cy;
+                if (!(dx * dx + dy * dy < P.r2)) break;
+                const int w = 4 * (B_POSRE + 2 * k);
+                c[0] = g.u53(w);
+                c[1] = g.u53(w + 2);
+                c[2] = g.u53(w + 4);
+                pos = add(S.o, mul(S.mat, v3(c[0], c[1], c[2])));
+            }
+        }
         dir = mul(S.rotG, draw_aniso(g, true));
         sign = dot(dir, S.gradT) < 0.0;
         sdom = e.index;
@@ -910,6 +981,12 @@ bool is_inside(const Domain& D, const Sdom& S, V3 p) {
         const Plane& P = D.planes[b];
         if (dot(P.n, p) + P.c < -S.eps) return false;
     }
+    if (S.pore >= 0) {
+        const Pore& Q = D.pores[S.pore];
+        const double dx = p.x - Q.cx, dy = p.y - Q.cy;
+        const double rr = Q.r - Q.eps;
+        if (dx * dx + dy * dy < rr * rr) return false;
+    }
     return true;
 }
 
@@ -926,6 +1003,23 @@ int advect(const Domain& D, const Sdom& S, Phonon& ph, double& dist) {
         if (t < minDistance) {
             minDistance = t;
             hit = b;
+        }
+    }
+    if (S.pore >= 0) {   // entering root of the ray with the pore's cylinder
+        const Pore& Q = D.pores[S.pore];
+        const double dx = ph.pos.x - Q.cx, dy = ph.pos.y - Q.cy;
+        const double bq = dx * ph.dir.x + dy * ph.dir.y;
+        if (bq < 0.0) {
+            const double aq = ph.dir.x * ph.dir.x + ph.dir.y * ph.dir.y;
+            const double cq = dx * dx + dy * dy - Q.r2;
+            const double disc = bq * bq - aq * cq;
+            if (disc >= 0.0) {
+                const double t = cq / (std::sqrt(disc) - bq);
+                if (t < minDistance) {
+                    minDistance = t;
+                    hit = -3;
+                }
+            }
         }
     }
     ph.scatNext -= minDistance;
@@ -1152,7 +1246,19 @@ void run_bundle(const Material& M, const Domain& D, const Problem& P, int64_t n,
         }
         accumulate(F, S, pre.pos, ph.pos, amount);
         EventRng g(P.seed, uint32_t(i + 1), uint64_t(n));
-        if (bdry >= 0) {
+        if (bdry == -3) {
+            ++nbdry;
+            const Pore& Q = D.pores[S.pore];
+            const V3 nn = normalized(v3((ph.pos.x - Q.cx) / Q.r, (ph.pos.y - Q.cy) / Q.r, 0.0));
+            int kind = Q.kind;
+            if (kind == MCG_PARTIAL) kind = g.word(W_SPEC) < Q.thr ? MCG_SPEC : MCG_DIFF;
+            if (kind == MCG_SPEC) {
+                ph.dir = normalized(mul(refl_matrix(nn), ph.dir));
+            } else {
+                ph.dir = normalized(mul(rot_unit(nn), draw_aniso(g, false)));
+                ph.nscat++;
+            }
+        } else if (bdry >= 0) {
             ++nbdry;
             const Plane& B = D.planes[bdry];
             int kind = B.kind;
@@ -1262,7 +1368,9 @@ void normalise(const Material& M, const Domain& D, const Problem& P, const Field
     }
     out.assign(R * Cn, 0.0);
     for (int64_t c = 0; c < Cn; ++c)
-        for (int64_t r = 0; r < R; ++r) out[c * R + r] = P.power * (sol[c * R + r] / D.cellVol[c]);
+        // a cell with no material (wholly inside a pore) reports zero
+        for (int64_t r = 0; r < R; ++r)
+            out[c * R + r] = D.cellVol[c] > 0.0 ? P.power * (sol[c * R + r] / D.cellVol[c]) : 0.0;
 }
 
 struct Built {
@@ -1403,6 +1511,15 @@ int oracle_sample_emission(const mcg_material_spec* mat, const mcg_domain_spec* 
         o[9] = double(ei);
     }
     return MCG_OK;
+}
+
+// Rectangle-disk overlap areas, element-wise: q holds 7 doubles per query
+// (x0, x1, y0, y1, cx, cy, r).
+void oracle_disk_area(const double* q, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double* a = q + 7 * i;
+        out[i] = mcg_disk::rect_in_disk(a[0], a[1], a[2], a[3], a[4], a[5], a[6]);
+    }
 }
 
 // The portable math functions, element-wise (for accuracy tests).

@@ -59,6 +59,8 @@ def lib():
         L.oracle_math.argtypes = [C.c_int32, C.POINTER(C.c_double), C.c_int64,
                                   C.POINTER(C.c_double)]
         L.oracle_math.restype = None
+        L.oracle_disk_area.argtypes = [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double)]
+        L.oracle_disk_area.restype = None
         _lib = L
     return _lib
 
@@ -169,3 +171,11 @@ def math_eval(fn, x):
     lib().oracle_math(fn, abi.dptr(x), x.size, abi.dptr(out))
     return out
 
+
+
+def disk_area(q):
+    """Area of rectangle [x0,x1]x[y0,y1] inside disk (cx,cy,r); q is (n, 7)."""
+    q = np.ascontiguousarray(q, np.float64).reshape(-1, 7)
+    out = np.zeros(len(q))
+    lib().oracle_disk_area(abi.dptr(q), len(q), abi.dptr(out))
+    return out

@@ -8,7 +8,7 @@ only differences are the order of FP64 additions).
 import numpy as np
 import pytest
 
-from montecarlocpp_amd import abi, engine, materials, scenes
+from montecarlocpp_amd import abi, engine, materials, scenes, workloads
 from oracle import oracle
 from tests.helpers import assert_counts_equal, rownorm_rel
 
@@ -116,6 +116,31 @@ def test_bundle_subrange(si300):
         _, _, cp = sc.solve(part, counts=True)
     assert np.array_equal(cf.nscat[5000:12000], cp.nscat)
     assert np.array_equal(cf.nstep[5000:12000], cp.nstep)
+
+
+@pytest.mark.parametrize("pore,walls", [(abi.DIFF, abi.DIFF), (abi.PARTIAL, abi.SPEC)])
+def test_porous_film_fp64(si300, pore, walls):
+    """Periodic porous film: a cylindrical pore in a 300 x 300 x 100 nm cell
+    periodic in x (gradient source) and y, 64 x 64 map."""
+    dom = scenes.porous_film(div=(64, 64, 0), pore=pore, walls=walls, specularity=0.4)
+    r = run_both(si300, dom, scenes.problem(20_000, maxscat=40, seed=13))
+    check_det(*r)
+
+
+def test_porous_film_fp32_within_3_sigma(si300):
+    """Production mode on the porous film: the effective conductivity from
+    FP32 flight agrees with the FP64 checker on independent seeds."""
+    dom = scenes.porous_film(div=(16, 16, 0))
+    kg, ko = [], []
+    with engine.Scene(si300, dom) as sc:
+        for s in range(8):
+            f = sc.solve(scenes.problem(20_000, maxscat=40, seed=s, precision=abi.FP32_FLIGHT))[0]
+            kg.append(workloads.k_eff_porous(f, engine.cell_volumes(dom)))
+    for s in range(8):
+        f = oracle.solve(si300, dom, scenes.problem(20_000, maxscat=40, seed=100 + s))[0]
+        ko.append(workloads.k_eff_porous(f, oracle.cell_volumes(dom)))
+    se = np.sqrt(np.var(kg, ddof=1) / 8 + np.var(ko, ddof=1) / 8)
+    assert abs(np.mean(kg) - np.mean(ko)) < 3 * se
 
 
 def test_fp32_production_within_3_sigma(si300):

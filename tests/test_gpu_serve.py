@@ -48,9 +48,10 @@ def test_alone_equals_mixed_equals_solve(precision):
 
 
 def test_budget_admission_and_cancel():
-    fac = serve.RequestFactory(nmin=1e4, nmax=1e5)
+    fac = serve.RequestFactory(nmin=1e4, nmax=1e5, families=("slab", "bar", "random_slab"))
     reqs = [fac.make(r) for r in range(1, 7)]
-    # each request holds ~125 KB of device memory: a 200 KB budget admits one
+    # each of these requests holds ~125 KB of device memory: a 200 KB budget
+    # admits one
     with engine.Server(budget_bytes=200_000, step_bundles=20_000) as sv:
         ids = [sv.submit(*r[1:]) for r in reqs]
         a, q, _ = sv.step()
