@@ -20,6 +20,10 @@ request stream through the continuous-batching server, with p50/p99
 latency), plus "porous_film" (the periodic porous film with a cylindrical
 pore, 1e8 bundles, 64 x 64 map, with its own CPU baseline sample).  --headline-only skips the secondary objects and the CPU leg.
 
+    python bench.py --only porous_film|batched|serving
+
+runs one secondary object alone (for profiling it).
+
     python bench.py --copy-probe
 
 runs only a 1 GiB device-to-device copy (torch): profiling it with
@@ -76,7 +80,14 @@ def main():
                     default=os.path.join(ROOT, "profiles", "trajectory_kernel_bar1e7_fp32_traffic.json"),
                     help="counter-measured HBM bytes per launch of the kernel (rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE, scaled by the copy-probe calibration)")
+    ap.add_argument("--only", choices=("porous_film", "batched", "serving"),
+                    help="run one secondary workload alone and print its JSON object")
     args = ap.parse_args()
+    if args.only:
+        run = {"porous_film": lambda: bench_porous(cpu=not args.no_cpu),
+               "batched": bench_batched, "serving": bench_serving}[args.only]
+        print(json.dumps({args.only: run()}))
+        return
     if args.copy_probe:
         import torch
         x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")

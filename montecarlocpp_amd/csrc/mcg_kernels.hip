@@ -202,6 +202,21 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
     L.id = n;
 }
 
+// Parameter along a segment of the crossing of grid plane n on one axis:
+// (n - b) / d.  FP64 divides, as the checker does (deterministic mode is
+// compared bit for bit); FP32 flight multiplies by the segment's reciprocal,
+// a rounding difference well inside its statistical comparison.
+template <typename F>
+__device__ __forceinline__ F crossing_param(int n, F b, F d, F inv) {
+    if constexpr (sizeof(F) == 8) {
+        (void)inv;
+        return (F(n) - b) / d;
+    } else {
+        (void)d;
+        return (F(n) - b) * inv;
+    }
+}
+
 // Field::accumulate (field.cpp:92-220) of `amt` (nv entries on rows r0..)
 // over the segment a -> b of subdomain S.
 template <typename F>
@@ -267,18 +282,20 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
     }
     // 2-D / 3-D: walk the crossings in parameter order; crossings of several
     // axes at exactly the same parameter are one step (the ordered map of
-    // field.cpp:156-218 merges equal keys).
-    F dc[3];
-    int pm[3], nxt[3], last[3];
-    bool live[3];
+    // field.cpp:156-218 merges equal keys).  pk holds each axis's next
+    // crossing parameter (2 once the axis has none left) and is recomputed
+    // only when that axis advances; the column index moves with the axes.
+    F dc[3], pk[3], inv[3];
+    int pm[3], nxt[3], last[3], dcol[3];
+    int col = S.col_off;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         dc[k] = ec[k] - bc[k];
-        live[k] = false;
-        pm[k] = 0;
-        nxt[k] = last[k] = 0;
+        col += bi[k] * S.stride[k];
+        pm[k] = nxt[k] = last[k] = dcol[k] = 0;
+        pk[k] = F(2);
+        inv[k] = F(0);
         if (fabs(double(dc[k])) < DBL_MIN || bi[k] == ei[k]) continue;
-        live[k] = true;
         if (bi[k] < ei[k]) {
             pm[k] = 1;
             nxt[k] = bi[k] + 1;
@@ -288,25 +305,14 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
             nxt[k] = bi[k];
             last[k] = ei[k] + 1;
         }
+        dcol[k] = pm[k] * S.stride[k];
+        inv[k] = F(1) / dc[k];
+        pk[k] = crossing_param<F>(nxt[k], bc[k], dc[k], inv[k]);
     }
-    int idx[3] = {bi[0], bi[1], bi[2]};
-    // parameter of each live axis's next crossing, recomputed only when that
-    // axis advances (same expression, so the same values as the map keys)
-    F pk[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) pk[k] = live[k] ? (F(nxt[k]) - bc[k]) / dc[k] : F(2);
     F prev = F(0);
     for (;;) {
-        F pmin = F(2);
-        bool any = false;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (!live[k]) continue;
-            if (!any || pk[k] < pmin) pmin = pk[k];
-            any = true;
-        }
-        if (!any) break;
-        const int col = S.col_off + idx[0] * S.stride[0] + idx[1] * S.stride[1] + idx[2] * S.stride[2];
+        const F pmin = fmin(fmin(pk[0], pk[1]), pk[2]);
+        if (pmin == F(2)) break;
         const double seg = double(pmin - prev);
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j)
@@ -314,15 +320,17 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
         prev = pmin;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            if (!live[k] || pk[k] != pmin) continue;
-            idx[k] += pm[k];
-            if (nxt[k] == last[k]) live[k] = false;
-            nxt[k] += pm[k];
-            if (live[k]) pk[k] = (F(nxt[k]) - bc[k]) / dc[k];
+            if (pk[k] != pmin) continue;
+            col += dcol[k];
+            if (nxt[k] == last[k]) {
+                pk[k] = F(2);
+            } else {
+                nxt[k] += pm[k];
+                pk[k] = crossing_param<F>(nxt[k], bc[k], dc[k], inv[k]);
+            }
         }
     }
     if (prev != F(1)) {
-        const int col = S.col_off + idx[0] * S.stride[0] + idx[1] * S.stride[1] + idx[2] * S.stride[2];
         const double seg = double(F(1) - prev);
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j)
