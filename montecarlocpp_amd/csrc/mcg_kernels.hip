@@ -42,6 +42,20 @@ constexpr size_t kLdsTallyCap = 48 * 1024;
 constexpr uint32_t kPosRedrawBlock = 64;
 constexpr uint32_t kMaxPosRedraw = 4096;
 
+// FP32 flight on scenes with 2-D / 3-D tallies defers the tally walks
+// (kernels with kDefer): a wavefront's lanes cross very different numbers of
+// cells per step (14 on average, 56 for the longest of 64 lanes on the
+// 64 x 64 porous-film map), so walking them to completion inside the step
+// leaves most lanes idle.  Deferred, a lane walks in the wavefront's walk
+// phase, which runs while at least kWalkQuorum lanes still walk; lanes whose
+// walks are done take their next step meanwhile.  FP64 (the deterministic
+// mode) and scenes without grids walk at once: the same additions, without
+// the walk state's registers held across steps.
+#ifndef MCG_WALK_QUORUM
+#define MCG_WALK_QUORUM 12
+#endif
+constexpr int kWalkQuorum = MCG_WALK_QUORUM;
+
 enum Counter : int { C_NEXT = 0, C_STEPS, C_ESC, C_ABS, C_TRUNC, C_BUNDLES, C_COUNT };
 
 template <typename F>
@@ -202,34 +216,80 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
     L.id = n;
 }
 
+// A 2-D / 3-D tally walk in progress: each axis's next crossing parameter
+// pk (2 once the axis has none left), the plane index of that crossing and
+// of the axis's last one, the column step of each axis, the current column,
+// the parameter reached and the step's amounts (exact in F: a sign times an
+// F value).  q is the segment's coordinate change (FP64) or its reciprocal
+// (FP32), see crossing_param.
+template <typename F>
+struct Walk {
+    F bc[3], q[3], pk[3];
+    int32_t nxt[3], last[3], dcol[3];
+    int32_t col, r0, nv;
+    F prev;
+    F amt[kMaxRows];
+};
+
 // Parameter along a segment of the crossing of grid plane n on one axis:
 // (n - b) / d.  FP64 divides, as the checker does (deterministic mode is
 // compared bit for bit); FP32 flight multiplies by the segment's reciprocal,
 // a rounding difference well inside its statistical comparison.
 template <typename F>
-__device__ __forceinline__ F crossing_param(int n, F b, F d, F inv) {
-    if constexpr (sizeof(F) == 8) {
-        (void)inv;
-        return (F(n) - b) / d;
-    } else {
-        (void)d;
-        return (F(n) - b) * inv;
-    }
+__device__ __forceinline__ F crossing_param(int n, F b, F q) {
+    if constexpr (sizeof(F) == 8) return (F(n) - b) / q;
+    else return (F(n) - b) * q;
 }
 
-// Field::accumulate (field.cpp:92-220) of `amt` (nv entries on rows r0..)
-// over the segment a -> b of subdomain S.
+// The crossings of the ordered map of field.cpp:156-218, one call per map
+// entry: crossings of several axes at exactly the same parameter are one
+// entry.  Adds the segment up to the next crossing to the current cell and
+// moves on; returns true after the last segment (up to parameter 1).
 template <typename F>
-__device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink,
+__device__ __forceinline__ bool walk_next(Walk<F>& W, const Sink& sink) {
+    const F pmin = fmin(fmin(W.pk[0], W.pk[1]), W.pk[2]);
+    const bool done = pmin == F(2);
+    const F upto = done ? F(1) : pmin;
+    if (!done || W.prev != F(1)) {
+        const double seg = double(upto - W.prev);
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j)
+            if (j < W.nv) sink.add(W.r0 + j, W.col, double(W.amt[j]) * seg);
+    }
+    if (done) return true;
+    W.prev = pmin;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (W.pk[k] != pmin) continue;
+        W.col += W.dcol[k];
+        if (W.nxt[k] == W.last[k]) {
+            W.pk[k] = F(2);
+        } else {
+            W.nxt[k] += W.dcol[k] > 0 ? 1 : -1;
+            W.pk[k] = crossing_param<F>(W.nxt[k], W.bc[k], W.q[k]);
+        }
+    }
+    return false;
+}
+
+// Field::accumulate (field.cpp:92-220) of the amounts `famt` (nv entries on
+// rows r0..) over the segment a -> b of subdomain S.  With kDefer a 2-D /
+// 3-D walk is only set up in W (the caller runs walk_next later) and the
+// function returns true; otherwise everything is added before it returns.
+template <typename F, bool kDefer>
+__device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink,
                                            const dev::V3<F>& a, const dev::V3<F>& b, int r0,
-                                           int nv, const double (&amt)[kMaxRows]) {
+                                           int nv, const F (&famt)[kMaxRows], Walk<F>& W) {
+    double amt[kMaxRows];
+#pragma unroll
+    for (int j = 0; j < kMaxRows; ++j) amt[j] = double(famt[j]);
     const int flag = S.accum;
-    if (flag < -1) return;
+    if (flag < -1) return false;
     if (flag < 0) {
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j)
             if (j < nv) sink.add(r0 + j, S.col_off, amt[j]);
-        return;
+        return false;
     }
     const dev::V3<F> la{a.x - S.o[0], a.y - S.o[1], a.z - S.o[2]};
     const dev::V3<F> lb{b.x - S.o[0], b.y - S.o[1], b.z - S.o[2]};
@@ -253,7 +313,7 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
 #pragma unroll
             for (int j = 0; j < kMaxRows; ++j)
                 if (j < nv) sink.add(r0 + j, S.col_off + bb, amt[j]);
-            return;
+            return false;
         }
         const double bcd = double(bcd_f), ecd = double(ecd_f);
         const double ad = fabs(double(ecd_f - bcd_f));
@@ -278,64 +338,40 @@ __device__ __forceinline__ void accumulate(const DevSdom<F>& S, const Sink& sink
             sink.add(r0 + j, S.col_off + ee, ca * fe);
             if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, ca);
         }
-        return;
+        return false;
     }
-    // 2-D / 3-D: walk the crossings in parameter order; crossings of several
-    // axes at exactly the same parameter are one step (the ordered map of
-    // field.cpp:156-218 merges equal keys).  pk holds each axis's next
-    // crossing parameter (2 once the axis has none left) and is recomputed
-    // only when that axis advances; the column index moves with the axes.
-    F dc[3], pk[3], inv[3];
-    int pm[3], nxt[3], last[3], dcol[3];
-    int col = S.col_off;
+    // 2-D / 3-D: the crossings in parameter order (walk_next)
+    W.col = S.col_off;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        dc[k] = ec[k] - bc[k];
-        col += bi[k] * S.stride[k];
-        pm[k] = nxt[k] = last[k] = dcol[k] = 0;
-        pk[k] = F(2);
-        inv[k] = F(0);
-        if (fabs(double(dc[k])) < DBL_MIN || bi[k] == ei[k]) continue;
+        const F dc = ec[k] - bc[k];
+        W.bc[k] = bc[k];
+        W.col += bi[k] * S.stride[k];
+        W.nxt[k] = W.last[k] = W.dcol[k] = 0;
+        W.pk[k] = F(2);
+        W.q[k] = F(0);
+        if (fabs(double(dc)) < DBL_MIN || bi[k] == ei[k]) continue;
         if (bi[k] < ei[k]) {
-            pm[k] = 1;
-            nxt[k] = bi[k] + 1;
-            last[k] = ei[k];
+            W.nxt[k] = bi[k] + 1;
+            W.last[k] = ei[k];
+            W.dcol[k] = S.stride[k];
         } else {
-            pm[k] = -1;
-            nxt[k] = bi[k];
-            last[k] = ei[k] + 1;
+            W.nxt[k] = bi[k];
+            W.last[k] = ei[k] + 1;
+            W.dcol[k] = -S.stride[k];
         }
-        dcol[k] = pm[k] * S.stride[k];
-        inv[k] = F(1) / dc[k];
-        pk[k] = crossing_param<F>(nxt[k], bc[k], dc[k], inv[k]);
+        W.q[k] = sizeof(F) == 8 ? dc : F(1) / dc;
+        W.pk[k] = crossing_param<F>(W.nxt[k], bc[k], W.q[k]);
     }
-    F prev = F(0);
-    for (;;) {
-        const F pmin = fmin(fmin(pk[0], pk[1]), pk[2]);
-        if (pmin == F(2)) break;
-        const double seg = double(pmin - prev);
+    W.prev = F(0);
+    W.r0 = r0;
+    W.nv = nv;
 #pragma unroll
-        for (int j = 0; j < kMaxRows; ++j)
-            if (j < nv) sink.add(r0 + j, col, amt[j] * seg);
-        prev = pmin;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (pk[k] != pmin) continue;
-            col += dcol[k];
-            if (nxt[k] == last[k]) {
-                pk[k] = F(2);
-            } else {
-                nxt[k] += pm[k];
-                pk[k] = crossing_param<F>(nxt[k], bc[k], dc[k], inv[k]);
-            }
-        }
+    for (int j = 0; j < kMaxRows; ++j) W.amt[j] = famt[j];
+    if (kDefer) return true;
+    while (!walk_next<F>(W, sink)) {
     }
-    if (prev != F(1)) {
-        const double seg = double(F(1) - prev);
-#pragma unroll
-        for (int j = 0; j < kMaxRows; ++j)
-            if (j < nv) sink.add(r0 + j, col, amt[j] * seg);
-    }
+    return false;
 }
 
 // Scatter off a cylindrical pore wall.  The wall normal at the hit point
@@ -483,10 +519,13 @@ struct LaneStats {
 
 // One scatter-step of the bundle in L (problem.cpp:401-435): advect, tally,
 // boundary or intrinsic scatter, termination test.  Returns the bundle's fate
-// when it ends, 0xFFFFFFFF while it lives.
-template <typename F>
+// when it ends, 0xFFFFFFFF while it lives.  `walking` is set when the step's
+// 2-D / 3-D tally walk was deferred to W (FP32 flight only).
+template <typename F, bool kDefer>
 __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View<F>& V,
-                                                 const Sink& sink, Lane<F>& L) {
+                                                 const Sink& sink, Lane<F>& L, Walk<F>& W,
+                                                 bool& walking) {
+    walking = false;
     const DevSdom<F>& S = V.sdoms[L.sdom];
     const F vel = V.vel[L.mode];
     // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
@@ -529,22 +568,23 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     if (dmin < -S.eps || !inside(V.planes, S, K.pores, L.pos)) {
         fate = MCG_ESCAPED;
     } else {
-        // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637)
-        const double sg = double(L.sign);
-        const double dt = sg * double(dmin / vel);
-        const double dx = sg * double(L.pos.x - pre.x);
-        const double dy = sg * double(L.pos.y - pre.y);
-        const double dz = sg * double(L.pos.z - pre.z);
+        // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637);
+        // each amount is a sign times an F value, exact in F and in FP64
+        const F sg = F(L.sign);
+        const F dt = sg * (dmin / vel);
+        const F dx = sg * (L.pos.x - pre.x);
+        const F dy = sg * (L.pos.y - pre.y);
+        const F dz = sg * (L.pos.z - pre.z);
         const int tk = K.tally;
         const bool with_t = tk == MCG_TEMP || tk == MCG_MULTI || tk == MCG_CUMTEMP;
-        const double amt[kMaxRows] = {with_t ? dt : dx, with_t ? dx : dy, with_t ? dy : dz, dz};
+        const F amt[kMaxRows] = {with_t ? dt : dx, with_t ? dx : dy, with_t ? dy : dz, dz};
         const int nv = tk == MCG_MULTI ? 4 : (with_t ? 1 : 3);
         int r0 = 0;
         if (tk >= MCG_CUMTEMP) {
             const int bin = int((int64_t(L.nscat) + K.step_cum - 1) / K.step_cum);
             r0 = tk == MCG_CUMTEMP ? bin : 3 * bin;
         }
-        accumulate<F>(S, sink, pre, L.pos, r0, nv, amt);
+        walking = accumulate<F, kDefer>(S, sink, pre, L.pos, r0, nv, amt, W);
 
         // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
         const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
@@ -633,6 +673,22 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     return fate;
 }
 
+// The wavefront's walk phase: every walking lane advances one crossing per
+// round.  The phase ends when no lane walks, or when fewer than kWalkQuorum
+// do while some other lane has a step to take or a bundle to claim (the
+// walkers resume in the next iteration's phase).  With nothing else left it
+// runs every walk to its end, so the wavefront's loop always progresses.
+template <typename F>
+__device__ __forceinline__ void walk_phase(Walk<F>& W, bool& walking, bool has_work, const Sink& sink) {
+    for (;;) {
+        const uint64_t m = __ballot(walking);
+        if (m == 0ull) break;
+        // lanes free to step or to claim a bundle once their walk is over
+        if (__popcll(m) < kWalkQuorum && __ballot(!walking && has_work) != 0ull) break;
+        if (walking && walk_next<F>(W, sink)) walking = false;
+    }
+}
+
 // Wave-level id pool: lanes whose bundle ended take the next ids of the pool,
 // refilled `chunk` ids at a time from *counter by one lane.  Returns the id
 // offset for lanes with `need` (all lanes call, the pool stays wave-uniform).
@@ -662,7 +718,7 @@ __device__ __forceinline__ int64_t claim(bool need, int lane, uint64_t lt_mask, 
 
 // Single solve: persistent grid, every lane refilled from the solve's global
 // bundle counter until the range [first, first + count) is exhausted.
-template <typename F, bool kLdsTables, bool kLdsTally>
+template <typename F, bool kLdsTables, bool kLdsTally, bool kDefer>
 __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int tid = threadIdx.x;
@@ -673,11 +729,12 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     const int lane = tid & (kWave - 1);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
     int64_t pool_base = 0, pool_lim = 0;
-    bool active = false, finished = false;
+    bool active = false, finished = false, walking = false;
     Lane<F> L;
+    Walk<F> W;
     LaneStats st;
     for (;;) {
-        const bool need = !active && !finished;
+        const bool need = !active && !finished && !walking;
         const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.counters + C_NEXT, K.chunk);
         if (need) {
             if (mine < K.count) {
@@ -687,13 +744,15 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
                 finished = true;
             }
         }
-        if (__ballot(active) == 0ull) break;
-        if (!active) continue;
-        const uint32_t fate = scatter_step<F>(K, V, sink, L);
-        if (fate != 0xFFFFFFFFu) {
-            st.add(fate, L.nstep);
-            active = false;
+        if (__ballot(active || walking) == 0ull) break;
+        if (active && !walking) {
+            const uint32_t fate = scatter_step<F, kDefer>(K, V, sink, L, W, walking);
+            if (fate != 0xFFFFFFFFu) {
+                st.add(fate, L.nstep);
+                active = false;
+            }
         }
+        if constexpr (kDefer) walk_phase<F>(W, walking, active || !finished, sink);
     }
     st.publish(K.counters, lane);
     if (kLdsTally) {
@@ -713,7 +772,7 @@ struct WorkItem {
 // order; when the item's solve differs from the one staged in its LDS it
 // flushes the staged tally and stages the new solve.  Within an item the
 // waves share the item's bundles through an LDS counter.
-template <typename F, bool kLdsTables, bool kLdsTally>
+template <typename F, bool kLdsTables, bool kLdsTally, bool kDefer>
 __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restrict__ solves,
                                                     const WorkItem* __restrict__ items, int nitems,
                                                     unsigned int* item_counter, int chunk) {
@@ -747,10 +806,11 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
         const KParams<F>& K = solves[cur];
         const Sink sink{V.direct, V.diff, K.cols};
         int64_t pool_base = 0, pool_lim = 0;
-        bool active = false, finished = false;
+        bool active = false, finished = false, walking = false;
         Lane<F> L;
+        Walk<F> W;
         for (;;) {
-            const bool need = !active && !finished;
+            const bool need = !active && !finished && !walking;
             const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, &s_next, chunk);
             if (need) {
                 if (mine < w.count) {
@@ -760,13 +820,15 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
                     finished = true;
                 }
             }
-            if (__ballot(active) == 0ull) break;
-            if (!active) continue;
-            const uint32_t fate = scatter_step<F>(K, V, sink, L);
-            if (fate != 0xFFFFFFFFu) {
-                st.add(fate, L.nstep);
-                active = false;
+            if (__ballot(active || walking) == 0ull) break;
+            if (active && !walking) {
+                const uint32_t fate = scatter_step<F, kDefer>(K, V, sink, L, W, walking);
+                if (fate != 0xFFFFFFFFu) {
+                    st.add(fate, L.nstep);
+                    active = false;
+                }
             }
+            if constexpr (kDefer) walk_phase<F>(W, walking, active || !finished, sink);
         }
         st.publish(K.counters, lane);
         __syncthreads();
@@ -777,6 +839,29 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
 }
 
 // ------------------------------------------------------------------ host
+enum class KernelKind { Trajectory, Batch };
+
+template <typename F, bool kTables, bool kTally, bool kDefer>
+const void* kernel_of(KernelKind kind) {
+    return kind == KernelKind::Trajectory
+               ? reinterpret_cast<const void*>(trajectory_kernel<F, kTables, kTally, kDefer>)
+               : reinterpret_cast<const void*>(batch_kernel<F, kTables, kTally, kDefer>);
+}
+
+// The kernel variant for an LDS plan; deferred walks only for FP32 flight on
+// scenes with 2-D / 3-D tallies.
+template <typename F>
+const void* pick_kernel(KernelKind kind, bool tables, bool tally, bool grid) {
+    if constexpr (sizeof(F) == 4) {
+        if (grid) {
+            return tables ? (tally ? kernel_of<F, true, true, true>(kind) : kernel_of<F, true, false, true>(kind))
+                          : (tally ? kernel_of<F, false, true, true>(kind) : kernel_of<F, false, false, true>(kind));
+        }
+    }
+    return tables ? (tally ? kernel_of<F, true, true, false>(kind) : kernel_of<F, true, false, false>(kind))
+                  : (tally ? kernel_of<F, false, true, false>(kind) : kernel_of<F, false, false, false>(kind));
+}
+
 thread_local std::string t_err;
 
 #define HIP_TRY(expr)                                                                   \
@@ -1022,10 +1107,7 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     const size_t lds = lp.bytes;
     if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     K.lds_tally = tally ? 1 : 0;
-    const void* fn = tables ? (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, true, true>)
-                                     : reinterpret_cast<const void*>(trajectory_kernel<F, true, false>))
-                            : (tally ? reinterpret_cast<const void*>(trajectory_kernel<F, false, true>)
-                                     : reinterpret_cast<const void*>(trajectory_kernel<F, false, false>));
+    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally, h.has_grid);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
     // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
@@ -1038,14 +1120,8 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, 0));
-    if (tables && tally)
-        hipLaunchKernelGGL((trajectory_kernel<F, true, true>), dim3(grid), dim3(block), lds, 0, K);
-    else if (tables)
-        hipLaunchKernelGGL((trajectory_kernel<F, true, false>), dim3(grid), dim3(block), lds, 0, K);
-    else if (tally)
-        hipLaunchKernelGGL((trajectory_kernel<F, false, true>), dim3(grid), dim3(block), lds, 0, K);
-    else
-        hipLaunchKernelGGL((trajectory_kernel<F, false, false>), dim3(grid), dim3(block), lds, 0, K);
+    void* args[] = {&K};
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, 0));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, 0));
     HIP_TRY(hipEventSynchronize(e1));
@@ -1089,6 +1165,7 @@ struct BatchPlan {
     std::vector<WorkItem> items;
     size_t lds = 0;
     bool tables = true, tally = true;
+    bool grid_tally = false;   // some solve tallies on a 2-D / 3-D grid
     int block = 256, grid = 0, chunk = 0, item = 0;
 };
 
@@ -1109,6 +1186,7 @@ int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
         grid = grid || h.has_grid;
     }
     const LdsPlan lp = plan_lds(geo, tab, tally_bytes, grid, !(tune && tune->lds_tables < 0), s0->lds_limit);
+    B.grid_tally = grid;
     B.tables = lp.tables;
     B.tally = lp.tally;
     B.lds = lp.bytes;
@@ -1160,20 +1238,12 @@ int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
 template <typename F>
 int launch_batch(const BatchPlan& B, const KParams<F>* kp, const WorkItem* items, int nitems,
                  unsigned int* item_ctr, hipStream_t stream) {
-    const void* fn = B.tables ? (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, true, true>)
-                                         : reinterpret_cast<const void*>(batch_kernel<F, true, false>))
-                              : (B.tally ? reinterpret_cast<const void*>(batch_kernel<F, false, true>)
-                                         : reinterpret_cast<const void*>(batch_kernel<F, false, false>));
+    const void* fn = pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
     const int grid = std::max(1, std::min(B.grid, nitems));
-    if (B.tables && B.tally)
-        hipLaunchKernelGGL((batch_kernel<F, true, true>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
-    else if (B.tables)
-        hipLaunchKernelGGL((batch_kernel<F, true, false>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
-    else if (B.tally)
-        hipLaunchKernelGGL((batch_kernel<F, false, true>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
-    else
-        hipLaunchKernelGGL((batch_kernel<F, false, false>), dim3(grid), dim3(B.block), B.lds, stream, kp, items, nitems, item_ctr, B.chunk);
+    int chunk = B.chunk;
+    void* args[] = {&kp, &items, &nitems, &item_ctr, &chunk};
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(B.block), args, B.lds, stream));
     HIP_TRY(hipGetLastError());
     return MCG_OK;
 }

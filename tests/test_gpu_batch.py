@@ -22,6 +22,8 @@ def test_batch_equals_individual_solves(precision, tune):
     cases.append((materials.si_like(250.0), scenes.film((2e-7, 6e-8, 2e-7), (8, 0, 0), 0.3),
                   scenes.problem(20_000, tally=abi.CUMFLUX, size=5, maxscat=21, seed=98,
                                  precision=precision)))
+    cases.append((materials.si_like(300.0), scenes.porous_film(div=(32, 32, 0)),
+                  scenes.problem(20_000, maxscat=30, seed=97, precision=precision)))
     print("batch of", len(cases), "tune", tune and (tune.block, tune.chunk), flush=True)
     sc = [engine.Scene(m, d) for m, d, _ in cases]
     probs = [p for _, _, p in cases]

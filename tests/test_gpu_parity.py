@@ -127,6 +127,31 @@ def test_porous_film_fp64(si300, pore, walls):
     check_det(*r)
 
 
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_FLIGHT])
+@pytest.mark.parametrize("div", [(64, 64, 0), (7, 5, 3)])
+def test_grid_walk_conserves_amounts(si300, precision, div):
+    """A tally grid only splits each step's amounts over the cells it
+    crosses: on the same streams, the volume-weighted sum of a grid tally
+    equals the single-cell tally of the same scene (FP32 flight walks its
+    2-D / 3-D tallies deferred, in the wavefront's walk phase)."""
+    grid = scenes.porous_film(div=div)
+    whole = scenes.porous_film(div=(0, 0, 0))
+    prob = scenes.problem(100_000, maxscat=60, seed=21, precision=precision)
+    with engine.Scene(si300, grid) as sc:
+        fg, sg, cg = sc.solve(prob, counts=True)
+    with engine.Scene(si300, whole) as sc:
+        fw, sw, cw = sc.solve(prob, counts=True)
+    assert_counts_equal(cg, cw)
+    vg, vw = engine.cell_volumes(grid), engine.cell_volumes(whole)
+    tot_g = fg @ vg
+    tot_w = fw @ vw
+    # deviational amounts carry both signs: bound the difference by the
+    # summed magnitudes of the cells, not by the (partly cancelled) total
+    scale = np.abs(fg) @ vg
+    tol = 2e-6 if precision == abi.FP32_FLIGHT else 1e-12
+    assert np.all(np.abs(tot_g - tot_w) <= tol * scale), (tot_g, tot_w, scale)
+
+
 def test_porous_film_fp32_within_3_sigma(si300):
     """Production mode on the porous film: the effective conductivity from
     FP32 flight agrees with the FP64 checker on independent seeds."""
