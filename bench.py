@@ -286,11 +286,11 @@ def bench_batched():
             "speedup_vs_single_solve_path": ts / tb}
 
 
-def bench_serving(lam=50.0, nreq=300):
+def bench_serving(lam=10.0, nreq=150):
     """Open-loop Poisson stream of slab / bar / random-slab / porous-film requests
     (1e4-1e8 bundles, T_ref 100-500 K) through the continuous-batching server."""
     from montecarlocpp_amd import serve
-    out = serve.run(lam, nreq, factory=serve.RequestFactory())
+    out = serve.run(lam, nreq, factory=serve.RequestFactory(), rerun_fraction=0.01)
     out["requests_mix"] = ("slab / bar / random slab / porous film, bundles log-uniform 1e4-1e8, "
                            "FP32 flight")
     return out

@@ -8,6 +8,9 @@
 //              film [dim0,2] [dim1] [div1] FilmDomain, dT = 1e6 dim0
 //              slab [L] [W] [div] [Thot] [Tcold]   isothermal walls, specular sides
 //              bar  [L] [W] [div] [Thot] [Tcold]   isothermal ends, diffuse sides
+//              porous [side] [thick] [radius] [div]  periodic porous film: side x
+//                         side x thick cell, centred pore, dT = 1 K across x,
+//                         diffuse pore, top and bottom, div x div map
 //   problem:   temp|flux|multi [nemit] [maxscat] [maxloop] [nsim]
 //              cumtemp|cumflux [nemit] [size] [maxscat] [maxloop] [nsim]
 //
@@ -22,13 +25,14 @@
 #include <chrono>
 #include <cstdlib>
 #include <ctime>
+#include <iomanip>
 
 #include "montecarlocpp.hpp"
 
 namespace {
 
 void dump_scene(const mcg_domain_spec& d) {
-    std::cout << "{\"cells\": [";
+    std::cout << std::setprecision(17) << "{\"cells\": [";
     for (int s = 0; s < d.nsdom; ++s) {
         const mcg_subdomain_spec& c = d.sdom[s];
         std::cout << (s ? ", " : "") << "{\"cell\": " << c.cell << ", \"origin\": [" << c.origin[0]
@@ -48,6 +52,12 @@ void dump_scene(const mcg_domain_spec& d) {
         std::cout << (k ? ", " : "") << "[" << p.sdom_a << ", " << p.face_a << ", " << p.sdom_b
                   << ", " << p.face_b << ", " << p.transl[0] << ", " << p.transl[1] << ", "
                   << p.transl[2] << "]";
+    }
+    std::cout << "], \"pores\": [";
+    for (int k = 0; k < d.npore; ++k) {
+        const mcg_pore_spec& q = d.pore[k];
+        std::cout << (k ? ", " : "") << "[" << q.sdom << ", " << q.kind << ", " << q.center[0] << ", "
+                  << q.center[1] << ", " << q.radius << ", " << q.specularity << "]";
     }
     std::cout << "]}" << std::endl;
 }
@@ -106,6 +116,11 @@ int main(int argc, const char* argv[]) {
                 dom.reset(new SlabDomain(Vector3d(L, W, W), Vector3l(div, 0, 0), th, tc));
             else
                 dom.reset(new BarDomain(Vector3d(L, W, W), Vector3l(div, 0, 0), th, tc));
+        } else if (domStr == "porous") {
+            double side, thick, radius;
+            long div;
+            argss >> side >> thick >> radius >> div;
+            dom.reset(new PorousFilmDomain<Diff>(Vector3d(side, side, thick), radius, Vector3l(div, div, 0), 1.0));
         } else {
             throw mcg::Error(MCG_EINVAL, "Invalid domain");
         }

@@ -579,6 +579,40 @@ public:
     }
 };
 
+// Periodic porous film (an extension: the reference's boundaries are planes).
+// One box cell periodic in x, with the volumetric gradient source -dT/dim0
+// of FilmDomain (domain.cpp:188-199), and in y; walls at z = 0 and z = dim2;
+// a cylindrical pore along z of radius r centred in the x-y cross-section.
+template <typename Wall = Diff>
+class PorousFilmDomain : public Domain {
+public:
+    using Sdom = Parallelepiped<PeriP, PeriP, Wall>;
+
+private:
+    Vector3d dim_;
+    Vector3l div_;
+    double radius_, dT_;
+    Sdom sdom_;
+
+public:
+    PorousFilmDomain(const Vector3d& dim, double radius, const Vector3l& div, double dT,
+                     int poreKind = MCG_DIFF, double specularity = 0.0)
+        : dim_(dim), div_(div), radius_(radius), dT_(dT),
+          sdom_(Vector3d::Zero(), Matrix3d::Diagonal(dim), div, Vector3d(-dT / dim(0), 0., 0.)) {
+        makePair(sdom_.template bdry<0>(), sdom_.template bdry<3>(), Vector3d(dim(0), 0., 0.));
+        makePair(sdom_.template bdry<1>(), sdom_.template bdry<4>(), Vector3d(0., dim(1), 0.));
+        addSdom(&sdom_);
+        addPore(0, 0.5 * dim(0), 0.5 * dim(1), radius, poreKind, specularity);
+    }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "PorousFilmDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2)
+           << "]\n  pore radius: " << radius_ << "\n  div: [" << div_(0) << " " << div_(1) << " "
+           << div_(2) << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+};
+
 // Cross-plane slab / bar with isothermal ends (the workloads' domains; the
 // reference ships none with IsotBoundary, the composition is API-legal).
 template <typename Side>

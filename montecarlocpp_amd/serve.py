@@ -67,8 +67,36 @@ class RequestFactory:
         return fam, self.material(T), dom, prob
 
 
-def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verbose=False):
-    """Serve nreq Poisson arrivals at rate lam; returns a metrics dict."""
+def _rownorm_rel(a, b):
+    out = []
+    for r in range(b.shape[0]):
+        nb = np.linalg.norm(b[r])
+        out.append(float(np.linalg.norm(a[r] - b[r]) / nb) if nb > 0 else float(np.linalg.norm(a[r])))
+    return out
+
+
+def rerun_errors(reqs, results, ids):
+    """Tally error of served requests against FP64 reruns of the same
+    problems (same seed and streams) through mcg_solve: row-norm relative
+    error of the temperature row and of the flux rows of the Multi tally."""
+    errs = []
+    for i in ids:
+        fam, mat, dom, prob = reqs[i]
+        p64 = scenes.problem(prob.nemit, tally=prob.tally, maxscat=prob.maxscat, maxloop=prob.maxloop,
+                             seed=prob.seed, precision=abi.FP64, size=prob.size)
+        with engine.Scene(mat, dom) as sc:
+            f64 = sc.solve(p64)[0]
+        e = _rownorm_rel(results[i], f64)
+        errs.append({"request": int(i + 1), "family": fam, "bundles": int(prob.nemit),
+                     "temperature": e[0], "heat_flux": max(e[1:])})
+    return errs
+
+
+def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verbose=False,
+        rerun_fraction=0.0):
+    """Serve nreq Poisson arrivals at rate lam; returns a metrics dict.  With
+    rerun_fraction > 0, that fraction of the requests (at least one) is
+    solved again in FP64 afterwards and the tally errors are reported."""
     factory = factory or RequestFactory()
     own = server is None
     server = server or engine.Server(step_bundles=step_bundles)
@@ -77,6 +105,11 @@ def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verb
     reqs = [factory.make(r + 1) for r in range(nreq)]   # built before the clock starts
     lat = np.full(nreq, np.nan)
     steps = np.zeros(nreq, np.int64)
+    sampled = set()
+    if rerun_fraction > 0:
+        k_rr = max(1, int(round(rerun_fraction * nreq)))
+        sampled = set(int(x) for x in rng.choice(nreq, size=k_rr, replace=False))
+    kept = {}
     outstanding = {}
     k = 0
     done = 0
@@ -97,6 +130,8 @@ def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verb
                     i = outstanding.pop(rid)
                     lat[i] = tnow - arrivals[i]
                     steps[i] = res[1]["steps"]
+                    if i in sampled:
+                        kept[i] = res[0]
                     done += 1
         elif k < nreq:
             time.sleep(max(0.0, min(arrivals[k] - now, 0.01)))
@@ -115,6 +150,8 @@ def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verb
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_mean_s": float(np.mean(lat)),
     }
+    if sampled:
+        out["fp64_rerun_errors"] = rerun_errors(reqs, kept, sorted(sampled))
     if verbose:
         print(json.dumps(out), flush=True)
     return out

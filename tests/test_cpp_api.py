@@ -45,7 +45,9 @@ def spec_dict(dom):
                                 for f in range(c.nface)]})
     pairs = [[d.pair[k].sdom_a, d.pair[k].face_a, d.pair[k].sdom_b, d.pair[k].face_b,
               *list(d.pair[k].transl)] for k in range(d.npair)]
-    return {"cells": cells, "pairs": pairs}
+    pores = [[d.pore[k].sdom, d.pore[k].kind, d.pore[k].center[0], d.pore[k].center[1],
+              d.pore[k].radius, d.pore[k].specularity] for k in range(d.npore)]
+    return {"cells": cells, "pairs": pairs, "pores": pores}
 
 
 def dump(tmp_path, args):
@@ -65,6 +67,7 @@ def deckdir(tmp_path):
     (["film", 2e-7, 5e-8, 6], lambda: scenes.film((2e-7, 5e-8, 2e-7), (0, 6, 0), 1e6 * 2e-7)),
     (["slab", 1e-7, 1e-7, 50, 1, -1], lambda: scenes.slab(1e-7, 50)),
     (["bar", 1e-6, 1e-7, 100, 10, -10], lambda: scenes.bar(1e-6, 1e-7, 1e-7, 100)),
+    (["porous", 3e-7, 1e-7, 1e-7, 64], lambda: scenes.porous_film()),
 ])
 def test_mirror_builds_the_same_scene(deckdir, args, py):
     got = dump(deckdir, args)
@@ -80,6 +83,10 @@ def test_mirror_builds_the_same_scene(deckdir, args, py):
     for g, w in zip(got["pairs"], want["pairs"]):
         assert g[:4] == w[:4]
         np.testing.assert_allclose(g[4:], w[4:], rtol=1e-12)
+    assert len(got["pores"]) == len(want["pores"])
+    for g, w in zip(got["pores"], want["pores"]):
+        assert g[:2] == w[:2]
+        np.testing.assert_allclose(g[2:], w[2:], rtol=1e-12)
 
 
 def test_driver_errors(deckdir):
@@ -106,6 +113,7 @@ def parse_solution(out):
     ["silicon", 300, "film", 2e-7, 5e-8, 6, "multi", 20000, 40, 0, 1],
     ["silicon", 300, "bar", 4e-7, 1e-7, 20, 10, -10, "multi", 20000, 1000000000, 0, 1],
     ["grey", 300, "bulk", 2e-7, 8, "cumflux", 10000, 4, 9, 0, 1],
+    ["silicon", 300, "porous", 3e-7, 1e-7, 1e-7, 16, "multi", 10000, 30, 0, 1],
 ])
 def test_driver_matches_checker(deckdir, deck):
     from oracle import oracle
@@ -122,6 +130,9 @@ def test_driver_matches_checker(deckdir, deck):
     elif kind == "bar":
         dom = scenes.bar(deck[3], deck[4], deck[4], deck[5], deck[6], deck[7])
         tally, nemit, maxscat, size = abi.MULTI, deck[9], deck[10], 0
+    elif kind == "porous":
+        dom = scenes.porous_film((deck[3], deck[3], deck[4]), deck[5], (deck[6], deck[6], 0))
+        tally, nemit, maxscat, size = abi.MULTI, deck[8], deck[9], 0
     else:
         dom = scenes.bulk((deck[3],) * 3, (deck[4], 0, 0), 1e6 * deck[3])
         tally, nemit, size, maxscat = abi.CUMFLUX, deck[6], deck[7], deck[8]

@@ -74,3 +74,28 @@ def test_open_loop_latency_report():
     assert out["requests"] == 40
     assert 0 < out["latency_p50_s"] <= out["latency_p99_s"]
     assert out["scatter_steps_per_s"] > 0
+
+
+def test_served_fp64_requests_equal_checker():
+    """Requests served in FP64 (deterministic mode), mixed in one server,
+    equal the CPU checker's solves of the same problems."""
+    from oracle import oracle
+    fac = serve.RequestFactory(nmin=1e4, nmax=4e4, precision=abi.FP64)
+    reqs = [fac.make(r) for r in range(1, 9)]
+    print("families", [r[0] for r in reqs], flush=True)
+    with engine.Server(step_bundles=60_000) as sv:
+        ids = [sv.submit(*r[1:]) for r in reqs]
+        res = drain(sv, ids)
+    for rid, (fam, mat, dom, prob) in zip(ids, reqs):
+        fo, so, _ = oracle.solve(mat, dom, prob)
+        assert res[rid][1]["steps"] == so["steps"], fam
+        assert np.all(rownorm_rel(res[rid][0], fo) < 1e-5), fam
+
+
+def test_rerun_sampling_reports_errors():
+    fac = serve.RequestFactory(nmin=1e4, nmax=1e5, families=("slab", "bar", "random_slab"))
+    out = serve.run(lam=400.0, nreq=20, factory=fac, rerun_fraction=0.1)
+    errs = out["fp64_rerun_errors"]
+    assert len(errs) == 2
+    for e in errs:
+        assert 0 <= e["heat_flux"] < 0.5 and 0 <= e["temperature"] < 0.5
