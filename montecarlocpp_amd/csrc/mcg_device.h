@@ -20,8 +20,10 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
                                         uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        // one 32 x 32 -> 64 multiply (v_mad_u64_u32) gives both halves
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t lo0 = uint32_t(p0), hi0 = uint32_t(p0 >> 32);
+        const uint32_t lo1 = uint32_t(p1), hi1 = uint32_t(p1 >> 32);
         const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
         c1 = lo1;
@@ -73,6 +75,8 @@ struct Real<double> {
     __device__ static __forceinline__ double recip_norm_scale(double x, double n2) {
         return x / sqrt(n2);
     }
+    // a / b, correctly rounded (the checker divides)
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
 };
 
 template <>
@@ -82,10 +86,16 @@ struct Real<float> {
     __device__ static __forceinline__ float uniform(uint32_t a, uint32_t) {
         return float(a >> 8) * 5.9604644775390625e-8f;
     }
-    __device__ static __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+    // FP32 flight is compared statistically, so it uses the hardware's
+    // one-instruction approximations: v_sqrt_f32, v_rcp_f32 (1 ulp) and
+    // v_sin_f32 / v_cos_f32 (argument in turns: sin(pi r) = sin(2 pi (r/2)),
+    // r/2 in [-1/2, 1/2), absolute error near 1e-6)
+    __device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ static __forceinline__ void sincos_pi(float r, float& s, float& c) {
-        sincospif(r, &s, &c);
+        s = __builtin_amdgcn_sinf(0.5f * r);
+        c = __builtin_amdgcn_cosf(0.5f * r);
     }
+    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     // -ln(1 - u) = -log2(1 - u) ln 2
     __device__ static __forceinline__ float neglog1m(float u) {
         return -__builtin_amdgcn_logf(1.0f - u) * 0.69314718055994531f;
@@ -100,16 +110,27 @@ struct V3 {
     F x, y, z;
 };
 
+// FP64 keeps the checker's separately rounded products and sums; FP32 flight
+// fuses them (v_fma_f32).
 template <typename F>
 __device__ __forceinline__ F dot3(const F* n, const V3<F>& v) {
-    return n[0] * v.x + n[1] * v.y + n[2] * v.z;
+    if constexpr (sizeof(F) == 4)
+        return __builtin_fmaf(n[2], v.z, __builtin_fmaf(n[1], v.y, n[0] * v.x));
+    else
+        return n[0] * v.x + n[1] * v.y + n[2] * v.z;
 }
 
 // column-major 3x3 times vector, row by row as (m0 x + m3 y) + m6 z
 template <typename F>
 __device__ __forceinline__ V3<F> matvec(const F* m, const V3<F>& v) {
-    return V3<F>{m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
-                 m[2] * v.x + m[5] * v.y + m[8] * v.z};
+    if constexpr (sizeof(F) == 4) {
+        return V3<F>{__builtin_fmaf(m[6], v.z, __builtin_fmaf(m[3], v.y, m[0] * v.x)),
+                     __builtin_fmaf(m[7], v.z, __builtin_fmaf(m[4], v.y, m[1] * v.x)),
+                     __builtin_fmaf(m[8], v.z, __builtin_fmaf(m[5], v.y, m[2] * v.x))};
+    } else {
+        return V3<F>{m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+                     m[2] * v.x + m[5] * v.y + m[8] * v.z};
+    }
 }
 
 // Phonon::dir normalises (phonon.cpp:88-93 with Eigen's normalized())

@@ -330,10 +330,13 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             lo = ee + 1;
             hi = bb - 1;
         }
+        // FP64 divides each amount as the checker does; FP32 flight
+        // multiplies by one reciprocal
+        const double rad = sizeof(F) == 8 ? 0.0 : 1.0 / ad;
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j) {
             if (j >= nv) continue;
-            const double ca = amt[j] / ad;
+            const double ca = sizeof(F) == 8 ? amt[j] / ad : amt[j] * rad;
             sink.add(r0 + j, S.col_off + bb, ca * fb);
             sink.add(r0 + j, S.col_off + ee, ca * fe);
             if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, ca);
@@ -535,7 +538,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         const DevPlane<F>& P = V.planes[b];
         const F nd = dev::dot3(P.n, L.dir);
         if (nd >= F(0)) continue;
-        const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
+        const F t = dev::Real<F>::div(-(P.c + dev::dot3(P.n, L.pos)), nd);
         if (t < dmin) {
             dmin = t;
             hit = b;
@@ -550,7 +553,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             const F cq = dx * dx + dy * dy - Q.r2;
             const F disc = bq * bq - aq * cq;
             if (disc >= F(0)) {
-                const F t = cq / (dev::Real<F>::sqrt_(disc) - bq);
+                const F t = dev::Real<F>::div(cq, dev::Real<F>::sqrt_(disc) - bq);
                 if (t < dmin) {
                     dmin = t;
                     hit = -3;
@@ -571,7 +574,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637);
         // each amount is a sign times an F value, exact in F and in FP64
         const F sg = F(L.sign);
-        const F dt = sg * (dmin / vel);
+        const F dt = sg * dev::Real<F>::div(dmin, vel);
         const F dx = sg * (L.pos.x - pre.x);
         const F dy = sg * (L.pos.y - pre.y);
         const F dz = sg * (L.pos.z - pre.z);
