@@ -51,6 +51,18 @@ constexpr uint32_t kMaxPosRedraw = 4096;
 // walks are done take their next step meanwhile.  FP64 (the deterministic
 // mode) and scenes without grids walk at once: the same additions, without
 // the walk state's registers held across steps.
+// Lanes whose bundle ended claim and emit new bundles together: emission is
+// a long stretch of code (three Philox blocks, the emitter search, the flux
+// alias draw, direction and free path), and run whenever any one lane of a
+// wavefront needs it (a bar bundle ends after 27 steps, so almost every
+// iteration) it would execute with two or three lanes active.  A wavefront
+// emits once at least kEmitQuorum lanes wait, or when no lane has anything
+// else to do.
+#ifndef MCG_EMIT_QUORUM
+#define MCG_EMIT_QUORUM 8
+#endif
+constexpr int kEmitQuorum = MCG_EMIT_QUORUM;
+
 #ifndef MCG_WALK_QUORUM
 #define MCG_WALK_QUORUM 12
 #endif
@@ -381,10 +393,11 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
 // points away from the axis (into the material); the reflection and the
 // Lambertian rotation are built from it as for a planar face.
 template <typename F>
-__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Event& ev, Lane<F>& L) {
+__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const uint4& b0, const uint4& b1,
+                                             Lane<F>& L) {
     const dev::V3<F> n = dev::normalized(dev::V3<F>{(L.pos.x - Q.cx) / Q.r, (L.pos.y - Q.cy) / Q.r, F(0)});
     int kind = Q.kind;
-    if (kind == MCG_PARTIAL) kind = ev.block(1).w < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
+    if (kind == MCG_PARTIAL) kind = b1.w < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
     if (kind == MCG_SPEC) {
         const F nv[3] = {n.x, n.y, n.z};
         F r[9];
@@ -406,7 +419,7 @@ __device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Eve
     const F r[9] = {F(1) - (tyy + tzz), txy + twz, txz - twy,
                     txy - twz, F(1) - (txx + tzz), tyz + twx,
                     txz + twy, tyz - twx, F(1) - (txx + tyy)};
-    L.dir = dev::normalized(dev::matvec(r, dev::draw_aniso<F>(ev.block(0), false)));
+    L.dir = dev::normalized(dev::matvec(r, dev::draw_aniso<F>(b0, false)));
     L.nscat++;
 }
 
@@ -537,9 +550,13 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     for (int b = S.pbeg; b < S.pend; ++b) {
         const DevPlane<F>& P = V.planes[b];
         const F nd = dev::dot3(P.n, L.dir);
-        if (nd >= F(0)) continue;
+        if constexpr (sizeof(F) == 8) {
+            if (nd >= F(0)) continue;
+        }
+        // FP32: every lane evaluates every plane (a reciprocal is cheaper
+        // than the divergence of skipping the planes it moves away from)
         const F t = dev::Real<F>::div(-(P.c + dev::dot3(P.n, L.pos)), nd);
-        if (t < dmin) {
+        if (nd < F(0) && t < dmin) {
             dmin = t;
             hit = b;
         }
@@ -590,9 +607,12 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         walking = accumulate<F, kDefer>(S, sink, pre, L.pos, r0, nv, amt, W);
 
         // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
+        // the event's first two Philox blocks serve every branch below; they
+        // are drawn once, outside the branches, so that a wavefront whose
+        // lanes take different branches evaluates them once
         const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
+        const uint4 b0 = ev.block(0), b1 = ev.block(1);
         if (hit == -1) {
-            const uint4 b0 = ev.block(0), b1 = ev.block(1);
             L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
             L.dir = dev::normalized(dev::draw_iso<F>(b0));
             L.nscat++;
@@ -601,13 +621,13 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             L.nbdry++;
             const DevPlane<F>& P = V.planes[hit];
             int kind = P.kind;
-            if (kind == MCG_PARTIAL) kind = ev.block(1).w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+            if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
             switch (kind) {
                 case MCG_SPEC:   // boundary.cpp:283-287
                     L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
                     break;
                 case MCG_DIFF:   // boundary.cpp:308-312
-                    L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(ev.block(0), false)));
+                    L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(b0, false)));
                     L.nscat++;
                     break;
                 case MCG_ISOT:   // boundary.cpp:456-460
@@ -640,7 +660,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             }
         } else {
             L.nbdry++;
-            pore_scatter<F>(K.pores[S.pore], ev, L);
+            pore_scatter<F>(K.pores[S.pore], b0, b1, L);
         }
         if (fate == 0xFFFFFFFFu) {
             if (int64_t(L.nscat) >= K.maxscat) fate = MCG_MAXSCAT;
@@ -676,17 +696,25 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     return fate;
 }
 
+// Whether the wavefront claims and emits this iteration (wave-uniform).
+__device__ __forceinline__ bool emit_now(bool need, bool busy) {
+    const uint64_t m = __ballot(need);
+    if (m == 0ull) return false;
+    return __popcll(m) >= kEmitQuorum || __ballot(busy) == 0ull;
+}
+
 // The wavefront's walk phase: every walking lane advances one crossing per
 // round.  The phase ends when no lane walks, or when fewer than kWalkQuorum
-// do while some other lane has a step to take or a bundle to claim (the
-// walkers resume in the next iteration's phase).  With nothing else left it
-// runs every walk to its end, so the wavefront's loop always progresses.
+// do while some other lane has a step to take (the walkers resume in the
+// next iteration's phase).  With no step to take it runs every walk to its
+// end, so the wavefront's loop always progresses (lanes waiting to emit do
+// not count: emission waits for the walkers).
 template <typename F>
 __device__ __forceinline__ void walk_phase(Walk<F>& W, bool& walking, bool has_work, const Sink& sink) {
     for (;;) {
         const uint64_t m = __ballot(walking);
         if (m == 0ull) break;
-        // lanes free to step or to claim a bundle once their walk is over
+        // lanes with a step to take once their walk is over
         if (__popcll(m) < kWalkQuorum && __ballot(!walking && has_work) != 0ull) break;
         if (walking && walk_next<F>(W, sink)) walking = false;
     }
@@ -738,13 +766,15 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     LaneStats st;
     for (;;) {
         const bool need = !active && !finished && !walking;
-        const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.counters + C_NEXT, K.chunk);
-        if (need) {
-            if (mine < K.count) {
-                emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
-                active = true;
-            } else {
-                finished = true;
+        if (emit_now(need, active || walking)) {
+            const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.counters + C_NEXT, K.chunk);
+            if (need) {
+                if (mine < K.count) {
+                    emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
+                    active = true;
+                } else {
+                    finished = true;
+                }
             }
         }
         if (__ballot(active || walking) == 0ull) break;
@@ -755,7 +785,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
                 active = false;
             }
         }
-        if constexpr (kDefer) walk_phase<F>(W, walking, active || !finished, sink);
+        if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
     }
     st.publish(K.counters, lane);
     if (kLdsTally) {
@@ -814,13 +844,15 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
         Walk<F> W;
         for (;;) {
             const bool need = !active && !finished && !walking;
-            const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, &s_next, chunk);
-            if (need) {
-                if (mine < w.count) {
-                    emit<F>(K, K.flux_alias, V.mfp, w.begin + mine, L);
-                    active = true;
-                } else {
-                    finished = true;
+            if (emit_now(need, active || walking)) {
+                const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, &s_next, chunk);
+                if (need) {
+                    if (mine < w.count) {
+                        emit<F>(K, K.flux_alias, V.mfp, w.begin + mine, L);
+                        active = true;
+                    } else {
+                        finished = true;
+                    }
                 }
             }
             if (__ballot(active || walking) == 0ull) break;
@@ -831,7 +863,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
                     active = false;
                 }
             }
-            if constexpr (kDefer) walk_phase<F>(W, walking, active || !finished, sink);
+            if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
         }
         st.publish(K.counters, lane);
         __syncthreads();
