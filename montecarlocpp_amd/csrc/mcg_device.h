@@ -133,6 +133,15 @@ __device__ __forceinline__ V3<F> matvec(const F* m, const V3<F>& v) {
     }
 }
 
+// Row r of matvec(m, v), evaluated exactly as matvec evaluates it.
+template <typename F>
+__device__ __forceinline__ F matrow(const F* m, int r, const V3<F>& v) {
+    if constexpr (sizeof(F) == 4)
+        return __builtin_fmaf(m[r + 6], v.z, __builtin_fmaf(m[r + 3], v.y, m[r] * v.x));
+    else
+        return m[r] * v.x + m[r + 3] * v.y + m[r + 6] * v.z;
+}
+
 // Phonon::dir normalises (phonon.cpp:88-93 with Eigen's normalized())
 template <typename F>
 __device__ __forceinline__ V3<F> normalized(const V3<F>& v) {

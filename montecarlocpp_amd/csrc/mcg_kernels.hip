@@ -34,6 +34,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kChunk = 256;            // bundle ids claimed per wavefront grab
 constexpr int kMaxRows = 4;            // nonzero amount entries per step
+constexpr int kMaxFaces = 6;           // faces of the largest cell (the parallelepiped)
 constexpr size_t kLdsTallyCap = 48 * 1024;
 // Emission inside a pored box: position redraw k uses Philox blocks
 // kPosRedrawBlock + 2k (two coordinates) and + 2k + 1 (the third); a pore
@@ -129,6 +130,15 @@ struct Lane {
     int64_t id;
 };
 
+// Whether p is outside the pore's cylinder (in the material), with the
+// subdomain's tolerance.
+template <typename F>
+__device__ __forceinline__ bool outside_pore(const DevPore<F>& Q, const dev::V3<F>& p) {
+    const F dx = p.x - Q.cx, dy = p.y - Q.cy;
+    const F rr = Q.r - Q.eps;
+    return !(dx * dx + dy * dy < rr * rr);
+}
+
 template <typename F>
 __device__ __forceinline__ bool inside(const DevPlane<F>* planes, const DevSdom<F>& S,
                                        const DevPore<F>* pores, const dev::V3<F>& p) {
@@ -136,13 +146,7 @@ __device__ __forceinline__ bool inside(const DevPlane<F>* planes, const DevSdom<
         const DevPlane<F>& P = planes[b];
         if (dev::dot3(P.n, p) + P.c < -S.eps) return false;
     }
-    if (S.pore >= 0) {
-        const DevPore<F>& Q = pores[S.pore];
-        const F dx = p.x - Q.cx, dy = p.y - Q.cy;
-        const F rr = Q.r - Q.eps;
-        if (dx * dx + dy * dy < rr * rr) return false;
-    }
-    return true;
+    return S.pore < 0 || outside_pore(pores[S.pore], p);
 }
 
 // Free path: l = v tau (-ln(1 - u)), redrawn while l < the smallest normal
@@ -305,6 +309,52 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
     }
     const dev::V3<F> la{a.x - S.o[0], a.y - S.o[1], a.z - S.o[2]};
     const dev::V3<F> lb{b.x - S.o[0], b.y - S.o[1], b.z - S.o[2]};
+    if (flag < 3) {
+        // one axis: only that coordinate, row `flag` of the inverse edge
+        // matrix with the same operations as the full product
+        const F bcd_f = S.divf[flag] * dev::matrow(S.inv, flag, la);
+        const F ecd_f = S.divf[flag] * dev::matrow(S.inv, flag, lb);
+        const int bb = min(max(int(floor(bcd_f)), 0), S.max[flag]);
+        const int ee = min(max(int(floor(ecd_f)), 0), S.max[flag]);
+        if (bb == ee) {
+#pragma unroll
+            for (int j = 0; j < kMaxRows; ++j)
+                if (j < nv) sink.add(r0 + j, S.col_off + bb, amt[j]);
+            return false;
+        }
+        const int lo = bb < ee ? bb + 1 : ee + 1;
+        const int hi = bb < ee ? ee - 1 : bb - 1;
+        if constexpr (sizeof(F) == 8) {
+            // the checker's FP64 arithmetic: each amount divided by |dc|
+            const double bcd = double(bcd_f), ecd = double(ecd_f);
+            const double ad = fabs(double(ecd_f - bcd_f));
+            const double fb = bb < ee ? double(1 + bb) - bcd : bcd - double(bb);
+            const double fe = bb < ee ? ecd - double(ee) : double(1 + ee) - ecd;
+#pragma unroll
+            for (int j = 0; j < kMaxRows; ++j) {
+                if (j >= nv) continue;
+                const double ca = amt[j] / ad;
+                sink.add(r0 + j, S.col_off + bb, ca * fb);
+                sink.add(r0 + j, S.col_off + ee, ca * fe);
+                if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, ca);
+            }
+        } else {
+            // FP32 flight: the per-step shares in FP32 (the amounts are FP32
+            // values), accumulated in FP64
+            const F rad = __builtin_amdgcn_rcpf(fabsf(ecd_f - bcd_f));
+            const F fb = bb < ee ? F(1 + bb) - bcd_f : bcd_f - F(bb);
+            const F fe = bb < ee ? ecd_f - F(ee) : F(1 + ee) - ecd_f;
+#pragma unroll
+            for (int j = 0; j < kMaxRows; ++j) {
+                if (j >= nv) continue;
+                const F ca = famt[j] * rad;
+                sink.add(r0 + j, S.col_off + bb, double(ca * fb));
+                sink.add(r0 + j, S.col_off + ee, double(ca * fe));
+                if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, double(ca));
+            }
+        }
+        return false;
+    }
     const dev::V3<F> ia = dev::matvec(S.inv, la), ib = dev::matvec(S.inv, lb);
     const F bc[3] = {S.divf[0] * ia.x, S.divf[1] * ia.y, S.divf[2] * ia.z};
     const F ec[3] = {S.divf[0] * ib.x, S.divf[1] * ib.y, S.divf[2] * ib.z};
@@ -314,46 +364,6 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         const int vb = int(floor(bc[k])), ve = int(floor(ec[k]));
         bi[k] = min(max(vb, 0), S.max[k]);
         ei[k] = min(max(ve, 0), S.max[k]);
-    }
-    if (flag < 3) {
-        // one axis: select it without indexing the arrays at run time
-        const F bcd_f = flag == 0 ? bc[0] : flag == 1 ? bc[1] : bc[2];
-        const F ecd_f = flag == 0 ? ec[0] : flag == 1 ? ec[1] : ec[2];
-        const int bb = flag == 0 ? bi[0] : flag == 1 ? bi[1] : bi[2];
-        const int ee = flag == 0 ? ei[0] : flag == 1 ? ei[1] : ei[2];
-        if (bb == ee) {
-#pragma unroll
-            for (int j = 0; j < kMaxRows; ++j)
-                if (j < nv) sink.add(r0 + j, S.col_off + bb, amt[j]);
-            return false;
-        }
-        const double bcd = double(bcd_f), ecd = double(ecd_f);
-        const double ad = fabs(double(ecd_f - bcd_f));
-        double fb, fe;
-        int lo, hi;
-        if (bb < ee) {
-            fb = double(1 + bb) - bcd;
-            fe = ecd - double(ee);
-            lo = bb + 1;
-            hi = ee - 1;
-        } else {
-            fb = bcd - double(bb);
-            fe = double(1 + ee) - ecd;
-            lo = ee + 1;
-            hi = bb - 1;
-        }
-        // FP64 divides each amount as the checker does; FP32 flight
-        // multiplies by one reciprocal
-        const double rad = sizeof(F) == 8 ? 0.0 : 1.0 / ad;
-#pragma unroll
-        for (int j = 0; j < kMaxRows; ++j) {
-            if (j >= nv) continue;
-            const double ca = sizeof(F) == 8 ? amt[j] / ad : amt[j] * rad;
-            sink.add(r0 + j, S.col_off + bb, ca * fb);
-            sink.add(r0 + j, S.col_off + ee, ca * fe);
-            if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, ca);
-        }
-        return false;
     }
     // 2-D / 3-D: the crossings in parameter order (walk_next)
     W.col = S.col_off;
@@ -547,18 +557,43 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
     F dmin = L.scat_next;
     int hit = -1;
-    for (int b = S.pbeg; b < S.pend; ++b) {
-        const DevPlane<F>& P = V.planes[b];
-        const F nd = dev::dot3(P.n, L.dir);
-        if constexpr (sizeof(F) == 8) {
-            if (nd >= F(0)) continue;
+    // FP32 keeps each face's signed distance and approach rate for the kill
+    // test below (at most kMaxFaces faces per cell, unrolled into registers)
+    F dist[kMaxFaces], rate[kMaxFaces];
+    if constexpr (sizeof(F) == 4) {
+        // every lane evaluates every face: a reciprocal is cheaper than the
+        // divergence of skipping the faces it moves away from
+#pragma unroll
+        for (int i = 0; i < kMaxFaces; ++i) {
+            dist[i] = F(1);
+            rate[i] = F(0);
+            const int b = S.pbeg + i;
+            if (b < S.pend) {
+                const DevPlane<F>& P = V.planes[b];
+                const F nd = dev::dot3(P.n, L.dir);
+                const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
+                                                                            __builtin_fmaf(P.n[0], L.pos.x, P.c)));
+                const F t = -d * __builtin_amdgcn_rcpf(nd);
+                if (nd < F(0) && t < dmin) {
+                    dmin = t;
+                    hit = b;
+                }
+                dist[i] = d;
+                rate[i] = nd;
+            }
         }
-        // FP32: every lane evaluates every plane (a reciprocal is cheaper
-        // than the divergence of skipping the planes it moves away from)
-        const F t = dev::Real<F>::div(-(P.c + dev::dot3(P.n, L.pos)), nd);
-        if (nd < F(0) && t < dmin) {
-            dmin = t;
-            hit = b;
+    } else {
+        (void)dist;
+        (void)rate;
+        for (int b = S.pbeg; b < S.pend; ++b) {
+            const DevPlane<F>& P = V.planes[b];
+            const F nd = dev::dot3(P.n, L.dir);
+            if (nd >= F(0)) continue;
+            const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
+            if (t < dmin) {
+                dmin = t;
+                hit = b;
+            }
         }
     }
     if (S.pore >= 0) {
@@ -585,7 +620,18 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     L.nstep++;
     const dev::V3<F> apos = L.pos;
     uint32_t fate = 0xFFFFFFFFu;
-    if (dmin < -S.eps || !inside(V.planes, S, K.pores, L.pos)) {
+    // kill test (subdomain.cpp:186-190): outside any face by more than eps
+    bool outside;
+    if constexpr (sizeof(F) == 4) {
+        // FP32: a face's distance after the move is d + dmin * rate
+        outside = false;
+#pragma unroll
+        for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -S.eps;
+        if (S.pore >= 0) outside |= !outside_pore(K.pores[S.pore], L.pos);
+    } else {
+        outside = !inside(V.planes, S, K.pores, L.pos);
+    }
+    if (dmin < -S.eps || outside) {
         fate = MCG_ESCAPED;
     } else {
         // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637);
