@@ -8,6 +8,8 @@
 //              film [dim0,2] [dim1] [div1] FilmDomain, dT = 1e6 dim0
 //              slab [L] [W] [div] [Thot] [Tcold]   isothermal walls, specular sides
 //              bar  [L] [W] [div] [Thot] [Tcold]   isothermal ends, diffuse sides
+//              jct [d0] [d3] | tee [d0] [d4] [div] | tube [d0] [d1] [d3] [div1] [div3]
+//                         the reference's assemblies (main.cpp:339-375)
 //              porous [side] [thick] [radius] [div]  periodic porous film: side x
 //                         side x thick cell, centred pore, dT = 1 K across x,
 //                         diffuse pore, top and bottom, div x div map
@@ -116,6 +118,23 @@ int main(int argc, const char* argv[]) {
                 dom.reset(new SlabDomain(Vector3d(L, W, W), Vector3l(div, 0, 0), th, tc));
             else
                 dom.reset(new BarDomain(Vector3d(L, W, W), Vector3l(div, 0, 0), th, tc));
+        } else if (domStr == "jct") {
+            // main.cpp:339-349: dim = (d0, d0, d0, d3), no divisions, dT = 2e6 d0
+            double d0, d3;
+            argss >> d0 >> d3;
+            dom.reset(new JctDomain({d0, d0, d0, d3}, {0, 0, 0, 0}, 2e6 * d0));
+        } else if (domStr == "tee") {
+            // main.cpp:350-363: dim = (d0 x4, d4), div = (n x4, 0), dT = 3e6 d0
+            double d0, d4;
+            long n;
+            argss >> d0 >> d4 >> n;
+            dom.reset(new TeeDomain({d0, d0, d0, d0, d4}, {n, n, n, n, 0}, 3e6 * d0));
+        } else if (domStr == "tube") {
+            // main.cpp:364-375: dim = (d0, d1, d1, d3), div = (0, n1, n1, n3), dT = 1e6 d0
+            double d0, d1, d3;
+            long n1, n3;
+            argss >> d0 >> d1 >> d3 >> n1 >> n3;
+            dom.reset(new TubeDomain({d0, d1, d1, d3}, {0, n1, n1, n3}, 1e6 * d0));
         } else if (domStr == "porous") {
             double side, thick, radius;
             long div;

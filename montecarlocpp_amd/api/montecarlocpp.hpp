@@ -252,10 +252,18 @@ public:
 using SpecBoundary = PlainBoundary<MCG_SPEC>;
 using DiffBoundary = PlainBoundary<MCG_DIFF>;
 
+// Order of makePair calls: a face's Inter partners are tried in this order
+// (boundary.cpp:349-359), so the flattened pair list keeps it.
+inline long nextPairSeq() {
+    static long seq = 0;
+    return ++seq;
+}
+
 class InterBoundary : public PlainBoundary<MCG_INTER> {
 public:
     using PlainBoundary<MCG_INTER>::PlainBoundary;
     std::vector<InterBoundary*> pairs;      // makePair partners added by this face
+    std::vector<long> pairSeq;              // their makePair call order
 };
 
 // Extension: specular with probability p, else diffuse.
@@ -287,6 +295,7 @@ public:
     int kind() const override { return MCG_PERI; }
     std::string type() const override { return "Peri"; }
     const PeriBase* pair = nullptr;
+    long pairSeq = 0;
     Vector3d transl;
     Matrix3d rot = Matrix3d::Identity();
 };
@@ -303,10 +312,14 @@ void makePair(PeriBoundary<S>& b1, PeriBoundary<S>& b2, const Vector3d& t,
               const Matrix3d& r = Matrix3d::Identity()) {
     if (b1.pair || b2.pair) throw mcg::Error(MCG_EINVAL, "Boundary already paired");
     b1.pair = &b2;
+    b1.pairSeq = nextPairSeq();
     b1.transl = t;
     b1.rot = r;
 }
-inline void makePair(InterBoundary& b1, InterBoundary& b2) { b1.pairs.push_back(&b2); }
+inline void makePair(InterBoundary& b1, InterBoundary& b2) {
+    b1.pairs.push_back(&b2);
+    b1.pairSeq.push_back(nextPairSeq());
+}
 
 // --------------------------------------------------------------- cells
 class Subdomain {
@@ -478,23 +491,30 @@ public:
     const mcg_domain_spec& spec() {
         cells_.clear();
         pairs_.clear();
+        std::vector<std::pair<long, mcg_pair_spec>> ordered;
         for (const Subdomain* s : sdomPtrs_) cells_.push_back(s->spec());
         for (size_t s = 0; s < sdomPtrs_.size(); ++s) {
             std::vector<const Boundary*> bp = sdomPtrs_[s]->bdryPtrs();
             for (size_t f = 0; f < bp.size(); ++f) {
                 std::vector<std::pair<const Boundary*, const Matrix3d*>> partners;
+                std::vector<long> seqs;
                 const Vector3d* t = nullptr;
                 if (bp[f]->kind() == MCG_PERI) {
                     auto* pb = static_cast<const PeriBase*>(bp[f]);
                     if (pb->pair) {
                         partners.push_back({pb->pair, &pb->rot});
+                        seqs.push_back(pb->pairSeq);
                         t = &pb->transl;
                     }
                 } else if (bp[f]->kind() == MCG_INTER) {
-                    for (const InterBoundary* q : static_cast<const InterBoundary*>(bp[f])->pairs)
-                        partners.push_back({q, nullptr});
+                    auto* ib = static_cast<const InterBoundary*>(bp[f]);
+                    for (size_t q = 0; q < ib->pairs.size(); ++q) {
+                        partners.push_back({ib->pairs[q], nullptr});
+                        seqs.push_back(ib->pairSeq[q]);
+                    }
                 }
-                for (auto& pr : partners) {
+                for (size_t q = 0; q < partners.size(); ++q) {
+                    auto& pr = partners[q];
                     mcg_pair_spec ps{};
                     ps.sdom_a = int32_t(s);
                     ps.face_a = int32_t(f);
@@ -505,10 +525,13 @@ public:
                         for (int k = 0; k < 3; ++k) ps.transl[k] = (*t)(k);
                     if (pr.second)
                         for (int i = 0; i < 9; ++i) ps.rot[i] = pr.second->m[i];
-                    pairs_.push_back(ps);
+                    ordered.push_back({seqs[q], ps});
                 }
             }
         }
+        std::stable_sort(ordered.begin(), ordered.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (auto& o : ordered) pairs_.push_back(o.second);
         spec_.nsdom = int32_t(cells_.size());
         spec_.npair = int32_t(pairs_.size());
         spec_.npore = int32_t(pores_.size());
@@ -575,6 +598,118 @@ public:
         std::ostringstream ss;
         ss << "FilmDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2)
            << "]\n  div: [" << div_(0) << " " << div_(1) << " " << div_(2) << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+};
+
+inline Matrix3d diag3(double a, double b, double c) { return Matrix3d::Diagonal(Vector3d(a, b, c)); }
+
+// JctDomain (domain.h:184-220, domain.cpp:338-383)
+class JctDomain : public Domain {
+    Vector3d g_;
+    Parallelepiped<PeriP, Spec, Diff, PeriP, Inter, Diff> s0_;
+    Parallelepiped<PeriP, Inter, Diff, Inter, Spec, Diff> s1_;
+    Parallelepiped<Inter, Inter, Diff, PeriP, Spec, Diff> s2_;
+    std::array<double, 4> dim_;
+    std::array<long, 4> div_;
+    double dT_;
+
+public:
+    JctDomain(const std::array<double, 4>& dim, const std::array<long, 4>& div, double dT)
+        : g_(-dT / (2. * dim[0]), 0., 0.),
+          s0_(Vector3d(0., 0., 0.), diag3(2. * dim[0], dim[1], dim[3]), Vector3l(2 * div[0], div[1], div[3]), g_),
+          s1_(Vector3d(0., dim[1], 0.), diag3(dim[0], dim[2], dim[3]), Vector3l(div[0], div[2], div[3]), g_),
+          s2_(Vector3d(dim[0], dim[1], 0.), diag3(dim[0], dim[2], dim[3]), Vector3l(div[0], div[2], div[3]), g_),
+          dim_(dim), div_(div), dT_(dT) {
+        makePair(s0_.bdry<4>(), s1_.bdry<1>());
+        makePair(s0_.bdry<4>(), s2_.bdry<1>());
+        makePair(s1_.bdry<3>(), s2_.bdry<0>());
+        const Vector3d transl(2. * dim[0], 0., 0.);
+        makePair(s0_.bdry<0>(), s0_.bdry<3>(), transl);
+        makePair(s1_.bdry<0>(), s2_.bdry<3>(), transl);
+        addSdom(&s0_);
+        addSdom(&s1_);
+        addSdom(&s2_);
+    }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "JctDomain " << this << "\n  dim: [" << dim_[0] << " " << dim_[1] << " " << dim_[2] << " "
+           << dim_[3] << "]\n  div: [" << div_[0] << " " << div_[1] << " " << div_[2] << " " << div_[3]
+           << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+};
+
+// TeeDomain (domain.h:222-259, domain.cpp:415-462)
+class TeeDomain : public Domain {
+    Vector3d g_;
+    Parallelepiped<PeriP, Diff, Spec, Inter, Diff, Spec> s0_;
+    Parallelepiped<Inter, Spec, Spec, Inter, Inter, Spec> s1_;
+    Parallelepiped<Diff, Inter, Spec, Diff, Spec, Spec> s2_;
+    Parallelepiped<Inter, Diff, Spec, PeriP, Diff, Spec> s3_;
+    std::array<double, 5> dim_;
+    std::array<long, 5> div_;
+    double dT_;
+
+public:
+    TeeDomain(const std::array<double, 5>& dim, const std::array<long, 5>& div, double dT)
+        : g_(-dT / (2. * dim[0] + dim[1]), 0., 0.),
+          s0_(Vector3d(0., 0., 0.), diag3(dim[0], dim[2], dim[4]), Vector3l(div[0], div[2], div[4]), g_),
+          s1_(Vector3d(dim[0], 0., 0.), diag3(dim[1], dim[2], dim[4]), Vector3l(div[1], div[2], div[4]), g_),
+          s2_(Vector3d(dim[0], dim[2], 0.), diag3(dim[1], dim[3], dim[4]), Vector3l(div[1], div[3], div[4]), g_),
+          s3_(Vector3d(dim[0] + dim[1], 0., 0.), diag3(dim[0], dim[2], dim[4]), Vector3l(div[0], div[2], div[4]),
+              g_),
+          dim_(dim), div_(div), dT_(dT) {
+        makePair(s0_.bdry<3>(), s1_.bdry<0>());
+        makePair(s1_.bdry<4>(), s2_.bdry<1>());
+        makePair(s1_.bdry<3>(), s3_.bdry<0>());
+        makePair(s0_.bdry<0>(), s3_.bdry<3>(), Vector3d(2. * dim[0] + dim[1], 0., 0.));
+        addSdom(&s0_);
+        addSdom(&s1_);
+        addSdom(&s2_);
+        addSdom(&s3_);
+    }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "TeeDomain " << this << "\n  dim: [" << dim_[0] << " " << dim_[1] << " " << dim_[2] << " "
+           << dim_[3] << " " << dim_[4] << "]\n  div: [" << div_[0] << " " << div_[1] << " " << div_[2] << " "
+           << div_[3] << " " << div_[4] << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+};
+
+// TubeDomain (domain.h:261-297, domain.cpp:495-540)
+class TubeDomain : public Domain {
+    Vector3d g_;
+    Parallelepiped<PeriP, Diff, Spec, PeriP, Diff, Inter> s0_;
+    Parallelepiped<PeriP, Inter, Inter, PeriP, Diff, Diff> s1_;
+    Parallelepiped<PeriP, Spec, Diff, PeriP, Inter, Diff> s2_;
+    std::array<double, 4> dim_;
+    std::array<long, 4> div_;
+    double dT_;
+
+public:
+    TubeDomain(const std::array<double, 4>& dim, const std::array<long, 4>& div, double dT)
+        : g_(-dT / dim[0], 0., 0.),
+          s0_(Vector3d(0., dim[1], 0.), diag3(dim[0], dim[3], dim[2]), Vector3l(div[0], div[3], div[2]), g_),
+          s1_(Vector3d(0., dim[1], dim[2]), diag3(dim[0], dim[3], dim[3]), Vector3l(div[0], div[3], div[3]), g_),
+          s2_(Vector3d(0., 0., dim[2]), diag3(dim[0], dim[1], dim[3]), Vector3l(div[0], div[1], div[3]), g_),
+          dim_(dim), div_(div), dT_(dT) {
+        makePair(s1_.bdry<1>(), s2_.bdry<4>());
+        makePair(s1_.bdry<2>(), s0_.bdry<5>());
+        const Vector3d transl(dim[0], 0., 0.);
+        makePair(s0_.bdry<0>(), s0_.bdry<3>(), transl);
+        makePair(s1_.bdry<0>(), s1_.bdry<3>(), transl);
+        makePair(s2_.bdry<0>(), s2_.bdry<3>(), transl);
+        addSdom(&s0_);
+        addSdom(&s1_);
+        addSdom(&s2_);
+    }
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "TubeDomain " << this << "\n  dim: [" << dim_[0] << " " << dim_[1] << " " << dim_[2] << " "
+           << dim_[3] << "]\n  div: [" << div_[0] << " " << div_[1] << " " << div_[2] << " " << div_[3]
+           << "]\n  dT:  " << dT_;
         return ss.str();
     }
 };

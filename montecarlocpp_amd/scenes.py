@@ -133,6 +133,67 @@ def film(dim, div, dT):
     return d
 
 
+def jct(dim, div, dT):
+    """JctDomain (domain.h:184-220, domain.cpp:338-383): a bar of length
+    2 dim0 along x (periodic through its own ends and through the two arms)
+    with a junction: two arms of width dim0 and length dim2 rise from its top
+    (y = dim1), joined to the bar's Rig face by one Inter face with two
+    partners, and to each other; thickness dim3, diffuse bottom and top."""
+    d0, d1, d2, d3 = dim
+    n0, n1, n2, n3 = div
+    g = (-dT / (2.0 * d0), 0.0, 0.0)
+    P, S, D, I = face(abi.PERI), face(abi.SPEC), face(abi.DIFF), face(abi.INTER)
+    d = Domain("jct")
+    s0 = d.add_box((0, 0, 0), (2 * d0, d1, d3), (2 * n0, n1, n3), [P, S, D, P, I, D], grad_t=g)
+    s1 = d.add_box((0, d1, 0), (d0, d2, d3), (n0, n2, n3), [P, I, D, I, S, D], grad_t=g)
+    s2 = d.add_box((d0, d1, 0), (d0, d2, d3), (n0, n2, n3), [I, I, D, P, S, D], grad_t=g)
+    d.pair(s0, RIG, s1, LEF)
+    d.pair(s0, RIG, s2, LEF)
+    d.pair(s1, FRO, s2, BAC)
+    d.pair(s0, BAC, s0, FRO, transl=(2 * d0, 0.0, 0.0))
+    d.pair(s1, BAC, s2, FRO, transl=(2 * d0, 0.0, 0.0))
+    return d
+
+
+def tee(dim, div, dT):
+    """TeeDomain (domain.h:222-259, domain.cpp:415-462): a periodic bar of
+    length 2 dim0 + dim1 along x in three boxes, with a stem of width dim1
+    and length dim3 on the middle box; width dim2, thickness dim4."""
+    d0, d1, d2, d3, d4 = dim
+    n0, n1, n2, n3, n4 = div
+    g = (-dT / (2.0 * d0 + d1), 0.0, 0.0)
+    P, S, D, I = face(abi.PERI), face(abi.SPEC), face(abi.DIFF), face(abi.INTER)
+    d = Domain("tee")
+    s0 = d.add_box((0, 0, 0), (d0, d2, d4), (n0, n2, n4), [P, D, S, I, D, S], grad_t=g)
+    s1 = d.add_box((d0, 0, 0), (d1, d2, d4), (n1, n2, n4), [I, S, S, I, I, S], grad_t=g)
+    s2 = d.add_box((d0, d2, 0), (d1, d3, d4), (n1, n3, n4), [D, I, S, D, S, S], grad_t=g)
+    s3 = d.add_box((d0 + d1, 0, 0), (d0, d2, d4), (n0, n2, n4), [I, D, S, P, D, S], grad_t=g)
+    d.pair(s0, FRO, s1, BAC)
+    d.pair(s1, RIG, s2, LEF)
+    d.pair(s1, FRO, s3, BAC)
+    d.pair(s0, BAC, s3, FRO, transl=(2 * d0 + d1, 0.0, 0.0))
+    return d
+
+
+def tube(dim, div, dT):
+    """TubeDomain (domain.h:261-297, domain.cpp:495-540): an L-shaped
+    periodic tube along x built from three boxes (two legs of widths dim1 and
+    dim2 around a dim3 x dim3 corner), each periodic through its own ends."""
+    d0, d1, d2, d3 = dim
+    n0, n1, n2, n3 = div
+    g = (-dT / d0, 0.0, 0.0)
+    P, S, D, I = face(abi.PERI), face(abi.SPEC), face(abi.DIFF), face(abi.INTER)
+    d = Domain("tube")
+    s0 = d.add_box((0, d1, 0), (d0, d3, d2), (n0, n3, n2), [P, D, S, P, D, I], grad_t=g)
+    s1 = d.add_box((0, d1, d2), (d0, d3, d3), (n0, n3, n3), [P, I, I, P, D, D], grad_t=g)
+    s2 = d.add_box((0, 0, d2), (d0, d1, d3), (n0, n1, n3), [P, S, D, P, I, D], grad_t=g)
+    d.pair(s1, LEF, s2, RIG)
+    d.pair(s1, BOT, s0, TOP)
+    for s in (s0, s1, s2):
+        d.pair(s, BAC, s, FRO, transl=(d0, 0.0, 0.0))
+    return d
+
+
 def porous_film(dim=(300e-9, 300e-9, 100e-9), radius=100e-9, div=(64, 64, 0), dT=1.0,
                 pore=abi.DIFF, walls=abi.DIFF, specularity=0.0):
     """Periodic porous film: a unit cell periodic in x (with the volumetric

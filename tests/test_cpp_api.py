@@ -68,6 +68,10 @@ def deckdir(tmp_path):
     (["slab", 1e-7, 1e-7, 50, 1, -1], lambda: scenes.slab(1e-7, 50)),
     (["bar", 1e-6, 1e-7, 100, 10, -10], lambda: scenes.bar(1e-6, 1e-7, 1e-7, 100)),
     (["porous", 3e-7, 1e-7, 1e-7, 64], lambda: scenes.porous_film()),
+    (["jct", 1e-7, 5e-8], lambda: scenes.jct((1e-7, 1e-7, 1e-7, 5e-8), (0, 0, 0, 0), 2e6 * 1e-7)),
+    (["tee", 1e-7, 5e-8, 3], lambda: scenes.tee((1e-7,) * 4 + (5e-8,), (3, 3, 3, 3, 0), 3e6 * 1e-7)),
+    (["tube", 2e-7, 1e-7, 5e-8, 4, 4],
+     lambda: scenes.tube((2e-7, 1e-7, 1e-7, 5e-8), (0, 4, 4, 4), 1e6 * 2e-7)),
 ])
 def test_mirror_builds_the_same_scene(deckdir, args, py):
     got = dump(deckdir, args)
@@ -114,6 +118,7 @@ def parse_solution(out):
     ["silicon", 300, "bar", 4e-7, 1e-7, 20, 10, -10, "multi", 20000, 1000000000, 0, 1],
     ["grey", 300, "bulk", 2e-7, 8, "cumflux", 10000, 4, 9, 0, 1],
     ["silicon", 300, "porous", 3e-7, 1e-7, 1e-7, 16, "multi", 10000, 30, 0, 1],
+    ["silicon", 300, "tube", 2e-7, 1e-7, 5e-8, 4, 4, "multi", 10000, 30, 0, 1],
 ])
 def test_driver_matches_checker(deckdir, deck):
     from oracle import oracle
@@ -129,6 +134,10 @@ def test_driver_matches_checker(deckdir, deck):
         tally, nemit, maxscat, size = abi.MULTI, deck[7], deck[8], 0
     elif kind == "bar":
         dom = scenes.bar(deck[3], deck[4], deck[4], deck[5], deck[6], deck[7])
+        tally, nemit, maxscat, size = abi.MULTI, deck[9], deck[10], 0
+    elif kind == "tube":
+        dom = scenes.tube((deck[3], deck[4], deck[4], deck[5]), (0, deck[6], deck[6], deck[7]),
+                          1e6 * deck[3])
         tally, nemit, maxscat, size = abi.MULTI, deck[9], deck[10], 0
     elif kind == "porous":
         dom = scenes.porous_film((deck[3], deck[3], deck[4]), deck[5], (deck[6], deck[6], 0))

@@ -137,3 +137,54 @@ def test_pair_rotation_map(mat):
     for lib in (oracle, engine):
         with pytest.raises(Exception, match="Rotation matrix incorrect"):
             lib.plan(mat, bad, scenes.problem(10))
+
+
+def test_assemblies_plan_and_checker_agree(mat):
+    """Jct / Tee / Tube (domain.cpp:338-540): emitter registry, weights,
+    counts and columns agree between the engine's builder and the checker,
+    and every bundle of a short solve stays inside (no escapes)."""
+    doms = [scenes.jct((1e-7, 1e-7, 1e-7, 5e-8), (3, 3, 3, 2), 2e6 * 1e-7),
+            scenes.tee((1e-7,) * 4 + (5e-8,), (3, 3, 3, 3, 0), 3e6 * 1e-7),
+            scenes.tube((2e-7, 1e-7, 1e-7, 5e-8), (0, 4, 4, 4), 1e6 * 2e-7)]
+    for d in doms:
+        prob = scenes.problem(5000, maxscat=20, seed=2)
+        io, wo, no = oracle.plan(mat, d, prob)
+        ie, we, ne = engine.plan(mat, d, prob)
+        np.testing.assert_allclose(we, wo, rtol=1e-15)
+        np.testing.assert_array_equal(ne, no)
+        assert io.cols == ie.cols and io.power == pytest.approx(ie.power, rel=1e-15)
+        np.testing.assert_allclose(engine.cell_volumes(d), oracle.cell_volumes(d), rtol=1e-14)
+        # volumes: the boxes' sum
+        vol = sum(np.prod(np.diag(np.array(c.mat).reshape(3, 3))) for c in d.cells)
+        assert oracle.cell_volumes(d).sum() == pytest.approx(vol, rel=1e-12)
+        _, st, _ = oracle.solve(mat, d, prob)
+        assert st["escapes"] == 0 and st["bundles"] == io.nemit
+
+
+def test_inter_faces_are_transparent(mat):
+    """A periodic box split in two by an Inter pair transports exactly like
+    the whole box: the same statistics of heat flux and temperature in the
+    same four cells (only the random streams differ, an Inter crossing being
+    a step of its own)."""
+    L, W = 2e-7, 6e-8
+    whole = scenes.Domain("whole")
+    s = whole.add_box((0, 0, 0), (L, W, W), (4, 0, 0),
+                      [f(abi.PERI), f(abi.DIFF), f(abi.SPEC), f(abi.PERI), f(abi.DIFF), f(abi.SPEC)],
+                      grad_t=(-1e6, 0.0, 0.0))
+    whole.pair(s, scenes.BAC, s, scenes.FRO, transl=(L, 0.0, 0.0))
+    split = scenes.Domain("split")
+    a = split.add_box((0, 0, 0), (L / 2, W, W), (2, 0, 0),
+                      [f(abi.PERI), f(abi.DIFF), f(abi.SPEC), f(abi.INTER), f(abi.DIFF), f(abi.SPEC)],
+                      grad_t=(-1e6, 0.0, 0.0))
+    b = split.add_box((L / 2, 0, 0), (L / 2, W, W), (2, 0, 0),
+                      [f(abi.INTER), f(abi.DIFF), f(abi.SPEC), f(abi.PERI), f(abi.DIFF), f(abi.SPEC)],
+                      grad_t=(-1e6, 0.0, 0.0))
+    split.pair(a, scenes.FRO, b, scenes.BAC)
+    split.pair(a, scenes.BAC, b, scenes.FRO, transl=(L, 0.0, 0.0))
+    fw = np.array([oracle.solve(mat, whole, scenes.problem(20000, maxscat=25, seed=s))[0]
+                   for s in range(10)])
+    fs = np.array([oracle.solve(mat, split, scenes.problem(20000, maxscat=25, seed=100 + s))[0]
+                   for s in range(10)])
+    se = np.sqrt(fw.var(axis=0, ddof=1) / 10 + fs.var(axis=0, ddof=1) / 10)
+    z = np.abs(fw.mean(axis=0) - fs.mean(axis=0)) / se
+    assert np.all(z[[0, 1]] < 4.5), z

@@ -118,6 +118,17 @@ def test_bundle_subrange(si300):
     assert np.array_equal(cf.nstep[5000:12000], cp.nstep)
 
 
+@pytest.mark.parametrize("name", ["jct", "tee", "tube"])
+def test_assemblies_fp64(si300, name):
+    """The reference's multi-box assemblies (domain.cpp:338-540): Inter faces
+    with one and two partners, periodic pairs between different boxes."""
+    dom = {"jct": lambda: scenes.jct((1e-7, 1e-7, 1e-7, 5e-8), (3, 3, 3, 2), 2e6 * 1e-7),
+           "tee": lambda: scenes.tee((1e-7,) * 4 + (5e-8,), (3, 3, 3, 3, 0), 3e6 * 1e-7),
+           "tube": lambda: scenes.tube((2e-7, 1e-7, 1e-7, 5e-8), (0, 4, 4, 4), 1e6 * 2e-7)}[name]()
+    r = run_both(si300, dom, scenes.problem(30_000, maxscat=40, seed=17))
+    check_det(*r)
+
+
 @pytest.mark.parametrize("pore,walls", [(abi.DIFF, abi.DIFF), (abi.PARTIAL, abi.SPEC)])
 def test_porous_film_fp64(si300, pore, walls):
     """Periodic porous film: a cylindrical pore in a 300 x 300 x 100 nm cell
