@@ -316,7 +316,7 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         const F ecd_f = S.divf[flag] * dev::matrow(S.inv, flag, lb);
         const int bb = min(max(int(floor(bcd_f)), 0), S.max[flag]);
         const int ee = min(max(int(floor(ecd_f)), 0), S.max[flag]);
-        if (bb == ee) {
+        if (sizeof(F) == 8 && bb == ee) {
 #pragma unroll
             for (int j = 0; j < kMaxRows; ++j)
                 if (j < nv) sink.add(r0 + j, S.col_off + bb, amt[j]);
@@ -340,7 +340,10 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             }
         } else {
             // FP32 flight: the per-step shares in FP32 (the amounts are FP32
-            // values), accumulated in FP64
+            // values), accumulated in FP64; a segment inside one cell takes
+            // the same path (its whole amount to that cell), so lanes do not
+            // split on it
+            const bool one = bb == ee;
             const F rad = __builtin_amdgcn_rcpf(fabsf(ecd_f - bcd_f));
             const F fb = bb < ee ? F(1 + bb) - bcd_f : bcd_f - F(bb);
             const F fe = bb < ee ? ecd_f - F(ee) : F(1 + ee) - ecd_f;
@@ -348,8 +351,8 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             for (int j = 0; j < kMaxRows; ++j) {
                 if (j >= nv) continue;
                 const F ca = famt[j] * rad;
-                sink.add(r0 + j, S.col_off + bb, double(ca * fb));
-                sink.add(r0 + j, S.col_off + ee, double(ca * fe));
+                sink.add(r0 + j, S.col_off + bb, double(one ? famt[j] : ca * fb));
+                if (!one) sink.add(r0 + j, S.col_off + ee, double(ca * fe));
                 if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, double(ca));
             }
         }
@@ -543,6 +546,64 @@ struct LaneStats {
     }
 };
 
+// The intrinsic and planar-face scatters of FP32 flight in one code path:
+// the isotropic (material.cpp:226-231) and Lambertian (boundary.cpp:308-312)
+// draws share the event's words and differ only in how the polar angle is
+// formed, and the new direction of a diffuse, specular or periodic face is
+// one product with the face's rotation, reflection or periodic map, so a
+// wavefront whose lanes take different events runs the direction code once.
+// FP64 keeps the checker's branch-by-branch arithmetic.
+template <typename F>
+__device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>& V, const DevSdom<F>& S,
+                                             const dev::Event& ev, const uint4& b0, const uint4& b1,
+                                             int hit, Lane<F>& L, uint32_t& fate) {
+    const bool intr = hit < 0;
+    const DevPlane<F>& P = V.planes[intr ? S.pbeg : hit];
+    int kind = intr ? -1 : P.kind;
+    if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+    const bool diff = kind == MCG_DIFF;
+    // direction: isotropic cos(theta) = r, or Lambertian sin^2(theta) = |r|
+    const F r = dev::Real<F>::uniform(b0.x, b0.y) * F(2) + F(-1);
+    const F ar = r < F(0) ? -r : r;
+    const F sinT = dev::Real<F>::sqrt_(intr ? F(1) - r * r : ar);
+    const F cosT = intr ? r : dev::Real<F>::sqrt_(F(1) - ar);
+    const F q = dev::Real<F>::uniform(b0.z, b0.w) * F(2) + F(-1);
+    F sq, cq;
+    dev::Real<F>::sincos_pi(q, sq, cq);
+    const dev::V3<F> v{sinT * cq, sinT * sq, cosT};
+    const bool turn = diff || kind == MCG_SPEC || kind == MCG_PERI;
+    const F* M = kind == MCG_SPEC ? P.refl : kind == MCG_PERI ? P.prot : P.rot;
+    const dev::V3<F> out = dev::matvec(M, diff ? v : L.dir);
+    if (intr || turn) L.dir = dev::normalized(intr ? v : out);
+    L.nscat += (intr || diff) ? 1u : 0u;
+    L.nbdry += intr ? 0u : 1u;
+    if (intr) {
+        L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
+        L.scat_next = draw_path<F>(ev, b1, V.mfp[L.mode]);
+    } else if (kind == MCG_PERI) {
+        const dev::V3<F> p = dev::matvec(P.prot, L.pos);
+        L.pos = dev::V3<F>{p.x + P.pt[0], p.y + P.pt[1], p.z + P.pt[2]};
+        L.sdom = P.target;
+    } else if (kind == MCG_ISOT) {
+        fate = MCG_ABSORBED;
+    } else if (kind == MCG_INTER) {
+        int target = -1;
+        if (P.icnt == 1) {
+            target = K.inter_list[P.ibeg];
+        } else {
+            for (int c = 0; c < P.icnt; ++c) {
+                const int cand = K.inter_list[P.ibeg + c];
+                if (inside(V.planes, V.sdoms[cand], K.pores, L.pos)) {
+                    target = cand;
+                    break;
+                }
+            }
+        }
+        if (target < 0) fate = MCG_ESCAPED;
+        else L.sdom = target;
+    }
+}
+
 // One scatter-step of the bundle in L (problem.cpp:401-435): advect, tally,
 // boundary or intrinsic scatter, termination test.  Returns the bundle's fate
 // when it ends, 0xFFFFFFFF while it lives.  `walking` is set when the step's
@@ -658,55 +719,64 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         // lanes take different branches evaluates them once
         const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
         const uint4 b0 = ev.block(0), b1 = ev.block(1);
-        if (hit == -1) {
-            L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
-            L.dir = dev::normalized(dev::draw_iso<F>(b0));
-            L.nscat++;
-            L.scat_next = draw_path<F>(ev, b1, V.mfp[L.mode]);
-        } else if (hit >= 0) {
-            L.nbdry++;
-            const DevPlane<F>& P = V.planes[hit];
-            int kind = P.kind;
-            if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
-            switch (kind) {
-                case MCG_SPEC:   // boundary.cpp:283-287
-                    L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
-                    break;
-                case MCG_DIFF:   // boundary.cpp:308-312
-                    L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(b0, false)));
-                    L.nscat++;
-                    break;
-                case MCG_ISOT:   // boundary.cpp:456-460
-                    fate = MCG_ABSORBED;
-                    break;
-                case MCG_PERI: { // boundary.cpp:517-522
-                    const dev::V3<F> q = dev::matvec(P.prot, L.pos);
-                    L.pos = dev::V3<F>{q.x + P.pt[0], q.y + P.pt[1], q.z + P.pt[2]};
-                    L.dir = dev::normalized(dev::matvec(P.prot, L.dir));
-                    L.sdom = P.target;
-                    break;
-                }
-                default: {       // MCG_INTER, boundary.cpp:349-359
-                    int target = -1;
-                    if (P.icnt == 1) {
-                        target = K.inter_list[P.ibeg];
-                    } else {
-                        for (int q = 0; q < P.icnt; ++q) {
-                            const int cand = K.inter_list[P.ibeg + q];
-                            if (inside(V.planes, V.sdoms[cand], K.pores, L.pos)) {
-                                target = cand;
-                                break;
-                            }
-                        }
-                    }
-                    if (target < 0) fate = MCG_ESCAPED;
-                    else L.sdom = target;
-                    break;
-                }
+        if constexpr (sizeof(F) == 4) {
+            if (hit != -3) {
+                scatter_fp32<F>(K, V, S, ev, b0, b1, hit, L, fate);
+            } else {
+                L.nbdry++;
+                pore_scatter<F>(K.pores[S.pore], b0, b1, L);
             }
         } else {
-            L.nbdry++;
-            pore_scatter<F>(K.pores[S.pore], b0, b1, L);
+            if (hit == -1) {
+                L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
+                L.dir = dev::normalized(dev::draw_iso<F>(b0));
+                L.nscat++;
+                L.scat_next = draw_path<F>(ev, b1, V.mfp[L.mode]);
+            } else if (hit >= 0) {
+                L.nbdry++;
+                const DevPlane<F>& P = V.planes[hit];
+                int kind = P.kind;
+                if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+                switch (kind) {
+                    case MCG_SPEC:   // boundary.cpp:283-287
+                        L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
+                        break;
+                    case MCG_DIFF:   // boundary.cpp:308-312
+                        L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(b0, false)));
+                        L.nscat++;
+                        break;
+                    case MCG_ISOT:   // boundary.cpp:456-460
+                        fate = MCG_ABSORBED;
+                        break;
+                    case MCG_PERI: { // boundary.cpp:517-522
+                        const dev::V3<F> q = dev::matvec(P.prot, L.pos);
+                        L.pos = dev::V3<F>{q.x + P.pt[0], q.y + P.pt[1], q.z + P.pt[2]};
+                        L.dir = dev::normalized(dev::matvec(P.prot, L.dir));
+                        L.sdom = P.target;
+                        break;
+                    }
+                    default: {       // MCG_INTER, boundary.cpp:349-359
+                        int target = -1;
+                        if (P.icnt == 1) {
+                            target = K.inter_list[P.ibeg];
+                        } else {
+                            for (int q = 0; q < P.icnt; ++q) {
+                                const int cand = K.inter_list[P.ibeg + q];
+                                if (inside(V.planes, V.sdoms[cand], K.pores, L.pos)) {
+                                    target = cand;
+                                    break;
+                                }
+                            }
+                        }
+                        if (target < 0) fate = MCG_ESCAPED;
+                        else L.sdom = target;
+                        break;
+                    }
+                }
+            } else {
+                L.nbdry++;
+                pore_scatter<F>(K.pores[S.pore], b0, b1, L);
+            }
         }
         if (fate == 0xFFFFFFFFu) {
             if (int64_t(L.nscat) >= K.maxscat) fate = MCG_MAXSCAT;
