@@ -309,7 +309,7 @@ int mcg_server_destroy(mcg_server* server);
 
 /* ------------------------------------------------------------- autotuner
  * Times candidate launch shapes (block 64-1024, sampling tables in LDS or
- * not, wavefront grab size or batch work-item size) on a bounded sample of
+ * not, bundle ids per wavefront grab) on a bounded sample of
  * the problem's bundles and returns the fastest; mcg_autotune also installs
  * it on the scene (mcg_scene_tune).  Only scheduling changes, never results.
  * sample_bundles <= 0 means min(nemit, 2^20); sample_fraction <= 0 means 0.1.

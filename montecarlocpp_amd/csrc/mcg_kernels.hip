@@ -92,6 +92,7 @@ struct KParams {
     double* direct;   // rows x cols, row-major
     double* diff;     // rows x (cols + 1), row-major
     unsigned long long* counters;
+    unsigned long long* next;   // bundle claim counter (offset into [first, first + count))
     uint32_t* c_nscat;
     uint32_t* c_nbdry;
     uint32_t* c_nstep;
@@ -883,7 +884,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     for (;;) {
         const bool need = !active && !finished && !walking;
         if (emit_now(need, active || walking)) {
-            const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.counters + C_NEXT, K.chunk);
+            const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
             if (need) {
                 if (mine < K.count) {
                     emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
@@ -910,61 +911,56 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     }
 }
 
-// One unit of batched work: bundles [begin, begin + count) of solve `solve`.
-struct WorkItem {
-    int32_t solve;
-    int32_t count;
-    int64_t begin;
-};
-
-// Many solves in one persistent launch.  A workgroup claims work items in
-// order; when the item's solve differs from the one staged in its LDS it
-// flushes the staged tally and stages the new solve.  Within an item the
-// waves share the item's bundles through an LDS counter.
+// Many solves in one persistent launch.  Workgroups start spread over the
+// solves and then visit every solve once, in order: a solve with bundles
+// left is staged in LDS (tables, faces, tally) and its bundles are claimed
+// from the solve's own counter, wavefront pool by wavefront pool, exactly as
+// trajectory_kernel claims them, until none are left; the workgroup then
+// flushes the solve's LDS tally and moves on.  A workgroup waits for its
+// slowest wavefront only when it leaves a solve.  Every solve is visited by
+// every workgroup, and a visit leaves the solve with no unclaimed bundles,
+// so all bundles are processed; a counter read that is stale only makes a
+// visit find nothing to do.
 template <typename F, bool kLdsTables, bool kLdsTally, bool kDefer>
-__global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restrict__ solves,
-                                                    const WorkItem* __restrict__ items, int nitems,
-                                                    unsigned int* item_counter, int chunk) {
+__global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restrict__ solves, int nsolves) {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int s_item;
-    __shared__ unsigned int s_next;
+    __shared__ int s_go, s_si;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-    int cur = -1;
-    View<F> V{};
-    LaneStats st;
-    for (;;) {
+    const int first = int((int64_t(blockIdx.x) * nsolves) / int64_t(gridDim.x));
+    for (int v = 0; v < nsolves; ++v) {
         if (tid == 0) {
-            s_item = int(atomicAdd(item_counter, 1u));
-            s_next = 0u;
+            const int si = first + v < nsolves ? first + v : first + v - nsolves;
+            s_si = si;
+            s_go = __hip_atomic_load(solves[si].next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                           (unsigned long long)solves[si].count ? 1 : 0;
         }
         __syncthreads();
-        const int item = s_item;
-        if (item >= nitems) break;
-        const WorkItem w = items[item];
-        if (w.solve != cur) {
-            if (cur >= 0 && kLdsTally)
-                flush_tally(V.direct, V.diff, solves[cur].direct, solves[cur].diff, solves[cur].rows,
-                            solves[cur].cols, solves[cur].lds_diff != 0, tid, blockDim.x);
-            __syncthreads();
-            V = stage<F, kLdsTables, kLdsTally>(solves[w.solve], lds, tid, blockDim.x);
-            cur = w.solve;
-            __syncthreads();
+        const bool go = s_go != 0;
+        // the solve index comes through LDS: the compiler then reads the
+        // solve's parameters per lane where it uses them, rather than holding
+        // all of them in scalar registers (which it cannot for a batch)
+        const KParams<F>& K = solves[s_si];
+        if (!go) {
+            __syncthreads();   // s_go is rewritten by the next visit
+            continue;
         }
-        const KParams<F>& K = solves[cur];
+        const View<F> V = stage<F, kLdsTables, kLdsTally>(K, lds, tid, blockDim.x);
         const Sink sink{V.direct, V.diff, K.cols};
+        __syncthreads();
         int64_t pool_base = 0, pool_lim = 0;
         bool active = false, finished = false, walking = false;
         Lane<F> L;
         Walk<F> W;
+        LaneStats st;
         for (;;) {
             const bool need = !active && !finished && !walking;
             if (emit_now(need, active || walking)) {
-                const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, &s_next, chunk);
+                const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
                 if (need) {
-                    if (mine < w.count) {
-                        emit<F>(K, K.flux_alias, V.mfp, w.begin + mine, L);
+                    if (mine < K.count) {
+                        emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
                         active = true;
                     } else {
                         finished = true;
@@ -983,10 +979,9 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
         }
         st.publish(K.counters, lane);
         __syncthreads();
+        if (kLdsTally) flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, K.lds_diff != 0, tid, blockDim.x);
+        __syncthreads();
     }
-    if (cur >= 0 && kLdsTally)
-        flush_tally(V.direct, V.diff, solves[cur].direct, solves[cur].diff, solves[cur].rows,
-                    solves[cur].cols, solves[cur].lds_diff != 0, tid, blockDim.x);
 }
 
 // ------------------------------------------------------------------ host
@@ -1236,6 +1231,7 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     K.direct = direct.p;
     K.diff = diff.p;
     K.counters = ctr.p;
+    K.next = ctr.p + C_NEXT;
     if (want_counts) {
         K.c_nscat = cnt.p;
         K.c_nbdry = cnt.p + p.count;
@@ -1309,15 +1305,14 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     return MCG_OK;
 }
 
-// Launch plan of a batch: per-solve tally offsets, work items, LDS layout.
+// Launch plan of a batch: per-solve tally offsets, LDS layout, launch shape.
 struct BatchPlan {
     std::vector<size_t> off_direct, off_diff;
     size_t n_direct = 0, n_diff = 0;
-    std::vector<WorkItem> items;
     size_t lds = 0;
     bool tables = true, tally = true;
     bool grid_tally = false;   // some solve tallies on a 2-D / 3-D grid
-    int block = 256, grid = 0, chunk = 0, item = 0;
+    int block = 256, grid = 0, chunk = kChunk;
 };
 
 // LDS layout, block size and grid of a batch launch over these solves.
@@ -1351,7 +1346,7 @@ int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
     per_cu = std::min(per_cu, std::max(1, 2048 / B.block));
     if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, int(tune->blocks_per_cu));
     B.grid = s0->num_cu * per_cu;
-    B.chunk = 64;
+    B.chunk = (tune && tune->chunk > 0) ? tune->chunk : kChunk;
     return MCG_OK;
 }
 
@@ -1372,29 +1367,18 @@ int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
         B.n_diff += size_t(P[i].rows) * (h.cols + 1);
         total += P[i].count;
     }
-    // work items: each solve's range cut into items of about total/(8 grid)
-    int64_t item = (tune && tune->chunk > 0) ? tune->chunk
-                                             : std::max<int64_t>(2048, total / (int64_t(B.grid) * 8));
-    item = std::min<int64_t>(item, 1 << 30);
-    B.item = int(item);
-    B.items.clear();
-    for (int i = 0; i < n; ++i)
-        for (int64_t b = 0; b < P[i].count; b += item)
-            B.items.push_back(WorkItem{i, int32_t(std::min<int64_t>(item, P[i].count - b)), P[i].first + b});
-    B.grid = int(std::min<int64_t>(B.grid, int64_t(B.items.size())));
+    // no more workgroups than there are bundles to give each lane one
+    B.grid = int(std::max<int64_t>(1, std::min<int64_t>(B.grid, (total + B.block - 1) / B.block)));
     return MCG_OK;
 }
 
 // Enqueue one batch launch on `stream`.
 template <typename F>
-int launch_batch(const BatchPlan& B, const KParams<F>* kp, const WorkItem* items, int nitems,
-                 unsigned int* item_ctr, hipStream_t stream) {
+int launch_batch(const BatchPlan& B, const KParams<F>* kp, int nsolves, hipStream_t stream) {
     const void* fn = pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
-    const int grid = std::max(1, std::min(B.grid, nitems));
-    int chunk = B.chunk;
-    void* args[] = {&kp, &items, &nitems, &item_ctr, &chunk};
-    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(B.block), args, B.lds, stream));
+    void* args[] = {&kp, &nsolves};
+    HIP_TRY(hipLaunchKernel(fn, dim3(std::max(1, B.grid)), dim3(B.block), args, B.lds, stream));
     HIP_TRY(hipGetLastError());
     return MCG_OK;
 }
@@ -1411,8 +1395,6 @@ int run_batch(int n, mcg_scene* const* scenes, const HostProblem* P, double* con
     DevBuf<unsigned long long> ctr;
     DevBuf<int64_t> cdf;
     DevBuf<KParams<F>> kp;
-    DevBuf<WorkItem> items;
-    DevBuf<unsigned int> item_ctr;
     std::vector<int64_t> cdf_all;
     std::vector<size_t> cdf_off(n);
     for (int i = 0; i < n; ++i) {
@@ -1423,8 +1405,6 @@ int run_batch(int n, mcg_scene* const* scenes, const HostProblem* P, double* con
     if ((rc = diff.alloc(B.n_diff))) return rc;
     if ((rc = ctr.alloc(size_t(C_COUNT) * n))) return rc;
     if ((rc = cdf.upload(cdf_all.data(), cdf_all.size()))) return rc;
-    if ((rc = items.upload(B.items.data(), B.items.size()))) return rc;
-    if ((rc = item_ctr.alloc(1))) return rc;
     std::vector<KParams<F>> K(n);
     for (int i = 0; i < n; ++i) {
         K[i] = KParams<F>{};
@@ -1433,18 +1413,19 @@ int run_batch(int n, mcg_scene* const* scenes, const HostProblem* P, double* con
         K[i].direct = direct.p + B.off_direct[i];
         K[i].diff = diff.p + B.off_diff[i];
         K[i].counters = ctr.p + size_t(C_COUNT) * i;
+        K[i].next = K[i].counters + C_NEXT;
+        K[i].chunk = B.chunk;
         K[i].lds_tally = B.tally ? 1 : 0;
     }
     if ((rc = kp.upload(K.data(), K.size()))) return rc;
     HIP_TRY(hipMemset(direct.p, 0, direct.n * sizeof(double)));
     HIP_TRY(hipMemset(diff.p, 0, diff.n * sizeof(double)));
     HIP_TRY(hipMemset(ctr.p, 0, ctr.n * sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(item_ctr.p, 0, sizeof(unsigned int)));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, 0));
-    if ((rc = launch_batch<F>(B, kp.p, items.p, int(B.items.size()), item_ctr.p, 0))) return rc;
+    if ((rc = launch_batch<F>(B, kp.p, n, 0))) return rc;
     HIP_TRY(hipEventRecord(e1, 0));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1514,10 +1495,10 @@ size_t scene_bytes(const HostScene& h) {
 
 // ====================================================================== serving
 // Continuous batching: requests are admitted as device memory allows, their
-// tallies and counters stay on the GPU across steps, and every step packs
-// work items of all active requests (fair shares of a per-step bundle budget)
-// into one persistent batch launch per precision.  Bundles never straddle
-// steps: an item runs its bundles to termination inside the step.
+// tallies and counters stay on the GPU across steps, and every step hands
+// each active request a fair share of a per-step bundle budget and runs all
+// the shares in one persistent batch launch per precision.  Bundles never
+// straddle steps: a share runs its bundles to termination inside the step.
 //
 // Steps are pipelined: a call plans and enqueues step k+1 (bundle cursors
 // advance when work is handed out), then waits for step k and retires the
@@ -1547,8 +1528,7 @@ struct Request {
 struct StepSlot {
     DevBuf<KParams<float>> kp32;
     DevBuf<KParams<double>> kp64;
-    DevBuf<WorkItem> it32, it64;
-    DevBuf<unsigned int> ictr;
+    DevBuf<unsigned long long> nx32, nx64;   // per-request bundle claim counters of the step
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool live = false;
     std::vector<std::pair<Request*, int64_t>> handed;
@@ -1641,7 +1621,7 @@ int retire(mcg_server* sv, Request* r) {
 template <typename F>
 int enqueue_group(mcg_server* sv, StepSlot& slot, const std::vector<Request*>& reqs,
                   const std::vector<int64_t>& take, DevBuf<KParams<F>>& kpbuf,
-                  DevBuf<WorkItem>& itbuf, unsigned int* ictr) {
+                  DevBuf<unsigned long long>& nxbuf) {
     const int n = int(reqs.size());
     std::vector<mcg_scene*> sc(n);
     std::vector<HostProblem> P(n);
@@ -1652,14 +1632,10 @@ int enqueue_group(mcg_server* sv, StepSlot& slot, const std::vector<Request*>& r
     BatchPlan B;
     int rc = plan_shape<F>(n, sc.data(), P.data(), &sv->tune, B);
     if (rc) return rc;
-    // items large enough that a workgroup's slowest bundle is a small part of
-    // its work, small enough that every CU gets items
     int64_t handed = 0;
     for (int64_t t : take) handed += t;
-    const int64_t item = sv->tune.chunk > 0
-                             ? sv->tune.chunk
-                             : std::max<int64_t>(8192, std::min<int64_t>(1 << 17, handed / (int64_t(B.grid) * 2)));
-    std::vector<WorkItem> items;
+    B.grid = int(std::max<int64_t>(1, std::min<int64_t>(B.grid, (handed + B.block - 1) / B.block)));
+    if (nxbuf.n < size_t(n) && (rc = nxbuf.alloc(size_t(n) * 2))) return rc;
     std::vector<KParams<F>> K(n);
     for (int i = 0; i < n; ++i) {
         Request* r = reqs[i];
@@ -1669,20 +1645,19 @@ int enqueue_group(mcg_server* sv, StepSlot& slot, const std::vector<Request*>& r
         K[i].direct = r->direct.p;
         K[i].diff = r->diff.p;
         K[i].counters = r->ctr.p;
+        K[i].next = nxbuf.p + i;
+        K[i].chunk = B.chunk;
+        K[i].first = r->P.first + r->cursor;   // this step's share of the request's bundles
+        K[i].count = take[i];
         K[i].lds_tally = B.tally ? 1 : 0;
-        for (int64_t b = 0; b < take[i]; b += item)
-            items.push_back(WorkItem{i, int32_t(std::min<int64_t>(item, take[i] - b)),
-                                     r->P.first + r->cursor + b});
         r->cursor += take[i];
         slot.handed.push_back({r, take[i]});
     }
     if (kpbuf.n < size_t(n) && (rc = kpbuf.alloc(size_t(n) * 2))) return rc;
-    if (itbuf.n < items.size() && (rc = itbuf.alloc(items.size() * 2))) return rc;
-    // synchronous copies: kpbuf / itbuf of this slot are not in use on the device
+    // synchronous copy: kpbuf of this slot is not in use on the device
     HIP_TRY(hipMemcpy(kpbuf.p, K.data(), K.size() * sizeof(KParams<F>), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(itbuf.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemsetAsync(ictr, 0, sizeof(unsigned int), sv->stream));
-    return launch_batch<F>(B, kpbuf.p, itbuf.p, int(items.size()), ictr, sv->stream);
+    HIP_TRY(hipMemsetAsync(nxbuf.p, 0, size_t(n) * sizeof(unsigned long long), sv->stream));
+    return launch_batch<F>(B, kpbuf.p, n, sv->stream);
 }
 
 // Hand out fair shares of the step budget and enqueue the step in `slot`.
@@ -1705,10 +1680,9 @@ int enqueue_step(mcg_server* sv, StepSlot& slot) {
     int rc;
     if (!slot.e0) HIP_TRY(hipEventCreate(&slot.e0));
     if (!slot.e1) HIP_TRY(hipEventCreate(&slot.e1));
-    if (!slot.ictr.p && (rc = slot.ictr.alloc(2))) return rc;
     HIP_TRY(hipEventRecord(slot.e0, sv->stream));
-    if (!g32.empty() && (rc = enqueue_group<float>(sv, slot, g32, t32, slot.kp32, slot.it32, slot.ictr.p))) return rc;
-    if (!g64.empty() && (rc = enqueue_group<double>(sv, slot, g64, t64, slot.kp64, slot.it64, slot.ictr.p + 1))) return rc;
+    if (!g32.empty() && (rc = enqueue_group<float>(sv, slot, g32, t32, slot.kp32, slot.nx32))) return rc;
+    if (!g64.empty() && (rc = enqueue_group<double>(sv, slot, g64, t64, slot.kp64, slot.nx64))) return rc;
     HIP_TRY(hipEventRecord(slot.e1, sv->stream));
     slot.live = true;
     return MCG_OK;
@@ -2023,11 +1997,11 @@ int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_sp
     double top_rate = -1.0;
     static const int blocks[] = {128, 256, 512, 1024};
     static const int tables[] = {1, -1};
-    static const int items[] = {0, 4096, 16384, 65536};
+    static const int chunks[] = {64, 256, 1024};
     for (int b : blocks)
         for (int t : tables)
-            for (int it : items) {
-                const mcg_tune cand{b, 1, it, 0, t};
+            for (int c : chunks) {
+                const mcg_tune cand{b, 1, c, 0, t};
                 double ms = 1e300;
                 int64_t steps = 0;
                 for (int rep = 0; rep < 2; ++rep) {
