@@ -497,8 +497,11 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
     if (kLdsTally) {
         const int n = K.rows * K.cols + (K.lds_diff ? K.rows * (K.cols + 1) : 0);
         for (int i = tid; i < n; i += nthreads) s_direct[i] = 0.0;
+        // both in LDS, so the compiler addresses them as LDS (ds_add_f64);
+        // without 1-D tallies the difference array is neither allocated nor
+        // used (range adds come only from 1-D tallies)
         v.direct = s_direct;
-        v.diff = K.lds_diff ? s_diff : K.diff;
+        v.diff = s_diff;
     } else {
         v.direct = K.direct;
         v.diff = K.diff;
