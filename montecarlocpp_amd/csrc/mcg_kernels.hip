@@ -312,7 +312,10 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
     const dev::V3<F> lb{b.x - S.o[0], b.y - S.o[1], b.z - S.o[2]};
     if (flag < 3) {
         // one axis: only that coordinate, row `flag` of the inverse edge
-        // matrix with the same operations as the full product
+        // matrix with the same operations as the full product.  The column
+        // offset is read once: the LDS atomics below could alias the staged
+        // subdomain as far as the compiler knows, and it would re-read it.
+        const int col0 = S.col_off;
         const F bcd_f = S.divf[flag] * dev::matrow(S.inv, flag, la);
         const F ecd_f = S.divf[flag] * dev::matrow(S.inv, flag, lb);
         const int bb = min(max(int(floor(bcd_f)), 0), S.max[flag]);
@@ -320,7 +323,7 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         if (sizeof(F) == 8 && bb == ee) {
 #pragma unroll
             for (int j = 0; j < kMaxRows; ++j)
-                if (j < nv) sink.add(r0 + j, S.col_off + bb, amt[j]);
+                if (j < nv) sink.add(r0 + j, col0 + bb, amt[j]);
             return false;
         }
         const int lo = bb < ee ? bb + 1 : ee + 1;
@@ -335,9 +338,9 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             for (int j = 0; j < kMaxRows; ++j) {
                 if (j >= nv) continue;
                 const double ca = amt[j] / ad;
-                sink.add(r0 + j, S.col_off + bb, ca * fb);
-                sink.add(r0 + j, S.col_off + ee, ca * fe);
-                if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, ca);
+                sink.add(r0 + j, col0 + bb, ca * fb);
+                sink.add(r0 + j, col0 + ee, ca * fe);
+                if (lo <= hi) sink.add_range(r0 + j, col0 + lo, col0 + hi, ca);
             }
         } else {
             // FP32 flight: the per-step shares in FP32 (the amounts are FP32
@@ -352,9 +355,9 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             for (int j = 0; j < kMaxRows; ++j) {
                 if (j >= nv) continue;
                 const F ca = famt[j] * rad;
-                sink.add(r0 + j, S.col_off + bb, double(one ? famt[j] : ca * fb));
-                if (!one) sink.add(r0 + j, S.col_off + ee, double(ca * fe));
-                if (lo <= hi) sink.add_range(r0 + j, S.col_off + lo, S.col_off + hi, double(ca));
+                sink.add(r0 + j, col0 + bb, double(one ? famt[j] : ca * fb));
+                if (!one) sink.add(r0 + j, col0 + ee, double(ca * fe));
+                if (lo <= hi) sink.add_range(r0 + j, col0 + lo, col0 + hi, double(ca));
             }
         }
         return false;
