@@ -113,6 +113,8 @@ struct Sink {
     __device__ __forceinline__ void add(int row, int col, double v) const {
         unsafeAtomicAdd(direct + row * cols + col, v);
     }
+    // add at a flat index row * cols + col
+    __device__ __forceinline__ void add_at(int idx, double v) const { unsafeAtomicAdd(direct + idx, v); }
     __device__ __forceinline__ void add_range(int row, int lo, int hi, double v) const {
         // +v on columns lo..hi through the difference array
         unsafeAtomicAdd(diff + row * (cols + 1) + lo, v);
@@ -243,7 +245,7 @@ template <typename F>
 struct Walk {
     F bc[3], q[3], pk[3];
     int32_t nxt[3], last[3], dcol[3];
-    int32_t col, r0, nv;
+    int32_t col, base, nv;   // base: the first row's flat offset, r0 * cols
     F prev;
     F amt[kMaxRows];
 };
@@ -268,10 +270,19 @@ __device__ __forceinline__ bool walk_next(Walk<F>& W, const Sink& sink) {
     const bool done = pmin == F(2);
     const F upto = done ? F(1) : pmin;
     if (!done || W.prev != F(1)) {
-        const double seg = double(upto - W.prev);
+        const int idx = W.base + W.col;
+        if constexpr (sizeof(F) == 8) {
+            const double seg = double(upto - W.prev);
 #pragma unroll
-        for (int j = 0; j < kMaxRows; ++j)
-            if (j < W.nv) sink.add(W.r0 + j, W.col, double(W.amt[j]) * seg);
+            for (int j = 0; j < kMaxRows; ++j)
+                if (j < W.nv) sink.add_at(idx + j * sink.cols, double(W.amt[j]) * seg);
+        } else {
+            // FP32 flight: the share in FP32, accumulated in FP64
+            const F seg = upto - W.prev;
+#pragma unroll
+            for (int j = 0; j < kMaxRows; ++j)
+                if (j < W.nv) sink.add_at(idx + j * sink.cols, double(W.amt[j] * seg));
+        }
     }
     if (done) return true;
     W.prev = pmin;
@@ -396,7 +407,7 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         W.pk[k] = crossing_param<F>(W.nxt[k], bc[k], W.q[k]);
     }
     W.prev = F(0);
-    W.r0 = r0;
+    W.base = r0 * sink.cols;
     W.nv = nv;
 #pragma unroll
     for (int j = 0; j < kMaxRows; ++j) W.amt[j] = famt[j];
