@@ -269,6 +269,13 @@ def bench_batched():
     _, st = engine.batch_solve(scs, probs)
     tb = time.perf_counter() - t0
     steps = sum(x["steps"] for x in st)
+    # the same batch captured once into a HIP graph and replayed
+    with engine.Batch(scs, probs) as gb:
+        gb.run()
+        t0 = time.perf_counter()
+        _, gst = gb.run()
+        tg = time.perf_counter() - t0
+    gsteps = sum(x["steps"] for x in gst)
     t0 = time.perf_counter()
     single_kernel_ms = 0.0
     for sc, p in zip(scs, probs):
@@ -281,6 +288,7 @@ def bench_batched():
             "bundles": int(sum(p.nemit for p in probs)), "scatter_steps": int(steps),
             "batched_steps_per_s": steps / tb, "batched_ms": tb * 1e3,
             "batched_kernel_ms": st[0]["kernel_ms"],
+            "graph_replay_steps_per_s": gsteps / tg, "graph_replay_ms": tg * 1e3,
             "single_solve_path_steps_per_s": steps / ts, "single_solve_path_ms": ts * 1e3,
             "single_solve_path_kernel_ms": single_kernel_ms,
             "speedup_vs_single_solve_path": ts / tb}

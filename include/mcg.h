@@ -271,6 +271,20 @@ int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec*
                     double* const* outs, const int64_t* out_lens,
                     mcg_stats* stats, const mcg_tune* tune);
 
+/* A batch captured once into a HIP graph and replayed: the scenes' device
+ * buffers, the tally and counter buffers and the launch are prepared by
+ * mcg_batch_create (all problems in one precision); each mcg_batch_run zeroes
+ * the tallies and runs the batch kernel by one hipGraphLaunch, optionally
+ * with new seeds (seeds[i] replaces problem i's seed; NULL keeps them), and
+ * returns what mcg_batch_solve returns for the same problems.  The scenes
+ * must outlive the batch; one host thread runs a batch at a time. */
+typedef struct mcg_batch mcg_batch;
+int mcg_batch_create(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
+                     const mcg_tune* tune, mcg_batch** out);
+int mcg_batch_run(mcg_batch* batch, const uint64_t* seeds, double* const* outs,
+                  const int64_t* out_lens, mcg_stats* stats);
+int mcg_batch_destroy(mcg_batch* batch);
+
 /* ---------------------------------------------------------------- serving
  * Continuous batching.  A request is one field solve (its own material,
  * domain and problem).  mcg_server_step admits queued requests in arrival

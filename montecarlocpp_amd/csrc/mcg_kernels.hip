@@ -1510,6 +1510,145 @@ size_t scene_bytes(const HostScene& h) {
 }  // namespace mcg
 
 
+// ================================================================ graph batch
+namespace mcg {
+
+// Clears a batch's tallies and counters inside its graph (a kernel rather
+// than hipMemsetAsync: memset nodes in captured graphs are avoided).
+__global__ void zero_kernel(double* a, size_t na, double* b, size_t nb, unsigned long long* c, size_t nc) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
+        if (i < na) a[i] = 0.0;
+        else if (i < na + nb) b[i - na] = 0.0;
+        else c[i - na - nb] = 0ull;
+    }
+}
+
+struct GraphBatchBase {
+    virtual ~GraphBatchBase() = default;
+    virtual int run(const uint64_t* seeds, double* const* outs, mcg_stats* stats) = 0;
+    virtual int64_t need(int i) const = 0;
+    int n = 0;
+};
+
+template <typename F>
+struct GraphBatch : GraphBatchBase {
+    std::vector<mcg_scene*> sc;
+    std::vector<HostProblem> P;
+    BatchPlan B;
+    DevBuf<double> direct, diff;
+    DevBuf<unsigned long long> ctr;
+    DevBuf<int64_t> cdf;
+    DevBuf<KParams<F>> kp;
+    std::vector<KParams<F>> K;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+
+    ~GraphBatch() override {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int64_t need(int i) const override { return P[i].rows * sc[i]->host.cols; }
+
+    int create(const mcg_tune* tune) {
+        HIP_TRY(hipSetDevice(sc[0]->device));
+        int rc = plan_batch<F>(n, sc.data(), P.data(), tune, B);
+        if (rc) return rc;
+        std::vector<int64_t> cdf_all;
+        std::vector<size_t> cdf_off(n);
+        for (int i = 0; i < n; ++i) {
+            cdf_off[i] = cdf_all.size();
+            cdf_all.insert(cdf_all.end(), P[i].emit_cdf.begin(), P[i].emit_cdf.end());
+        }
+        if ((rc = direct.alloc(B.n_direct))) return rc;
+        if ((rc = diff.alloc(B.n_diff))) return rc;
+        if ((rc = ctr.alloc(size_t(C_COUNT) * n))) return rc;
+        if ((rc = cdf.upload(cdf_all.data(), cdf_all.size()))) return rc;
+        K.resize(n);
+        for (int i = 0; i < n; ++i) {
+            K[i] = KParams<F>{};
+            fill_params<F>(sc[i], P[i], K[i]);
+            K[i].emit_cdf = cdf.p + cdf_off[i];
+            K[i].direct = direct.p + B.off_direct[i];
+            K[i].diff = diff.p + B.off_diff[i];
+            K[i].counters = ctr.p + size_t(C_COUNT) * i;
+            K[i].next = K[i].counters + C_NEXT;
+            K[i].chunk = B.chunk;
+            K[i].lds_tally = B.tally ? 1 : 0;
+        }
+        if ((rc = kp.upload(K.data(), K.size()))) return rc;
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        // the kernel's LDS limit is an attribute of the function, set before capture
+        HIP_TRY(hipFuncSetAttribute(pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const size_t total = direct.n + diff.n + ctr.n;
+        const int zblocks = int(std::min<size_t>(1024, (total + 255) / 256));
+        hipLaunchKernelGGL(zero_kernel, dim3(std::max(1, zblocks)), dim3(256), 0, stream, direct.p, direct.n,
+                           diff.p, diff.n, ctr.p, ctr.n);
+        rc = launch_batch<F>(B, kp.p, n, stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return set_error(MCG_EDEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        graph = g;
+        HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return MCG_OK;
+    }
+
+    int run(const uint64_t* seeds, double* const* outs, mcg_stats* stats) override {
+        auto t0 = std::chrono::steady_clock::now();
+        HIP_TRY(hipSetDevice(sc[0]->device));
+        if (seeds) {
+            for (int i = 0; i < n; ++i) {
+                K[i].k0 = uint32_t(seeds[i]);
+                K[i].k1 = uint32_t(seeds[i] >> 32);
+            }
+            HIP_TRY(hipMemcpy(kp.p, K.data(), K.size() * sizeof(KParams<F>), hipMemcpyHostToDevice));
+        }
+        HIP_TRY(hipEventRecord(e0, stream));
+        HIP_TRY(hipGraphLaunch(exec, stream));
+        HIP_TRY(hipEventRecord(e1, stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        std::vector<double> hd(direct.n), hf(diff.n);
+        std::vector<unsigned long long> hc(ctr.n);
+        HIP_TRY(hipMemcpy(hd.data(), direct.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(hf.data(), diff.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(hc.data(), ctr.p, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int i = 0; i < n; ++i) {
+            finalise_field(sc[i]->host, P[i], hd.data() + B.off_direct[i], hf.data() + B.off_diff[i], outs[i]);
+            const unsigned long long* c = hc.data() + size_t(C_COUNT) * i;
+            if (stats) {
+                stats[i].bundles = int64_t(c[C_BUNDLES]);
+                stats[i].steps = int64_t(c[C_STEPS]);
+                stats[i].escapes = int64_t(c[C_ESC]);
+                stats[i].absorbed = int64_t(c[C_ABS]);
+                stats[i].truncated = int64_t(c[C_TRUNC]);
+                stats[i].kernel_ms = ms;
+                stats[i].wall_ms = wall;
+            }
+            if (int64_t(c[C_BUNDLES]) != P[i].count)
+                return set_error(MCG_EDEVICE, "batch finished with bundles unaccounted for");
+        }
+        return MCG_OK;
+    }
+};
+
+}  // namespace mcg
+
 // ====================================================================== serving
 // Continuous batching: requests are admitted as device memory allows, their
 // tallies and counters stay on the GPU across steps, and every step hands
@@ -1944,6 +2083,62 @@ int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec*
         if (stats)
             for (size_t k = 0; k < idx.size(); ++k) stats[idx[k]] = ss[k];
     }
+    return MCG_OK;
+}
+
+struct mcg_batch {
+    std::unique_ptr<mcg::GraphBatchBase> impl;
+};
+
+int mcg_batch_create(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,
+                     const mcg_tune* tune, mcg_batch** out) {
+    if (!out || n <= 0 || !scenes || !probs) return mcg::set_error(MCG_EINVAL, "NULL argument or empty batch");
+    *out = nullptr;
+    std::vector<mcg::HostProblem> P(n);
+    for (int32_t i = 0; i < n; ++i) {
+        if (!scenes[i]) return mcg::set_error(MCG_EINVAL, "NULL scene");
+        if (scenes[i]->device != scenes[0]->device) return mcg::set_error(MCG_EINVAL, "batch scenes on different devices");
+        int rc = mcg::build_problem(&probs[i], scenes[i]->host, P[i]);
+        if (rc) return rc;
+        if (P[i].precision != P[0].precision) return mcg::set_error(MCG_EINVAL, "a graph batch takes one precision");
+    }
+    std::unique_ptr<mcg_batch> b(new (std::nothrow) mcg_batch());
+    if (!b) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+    int rc;
+    if (P[0].precision == MCG_FP64) {
+        auto* g = new (std::nothrow) mcg::GraphBatch<double>();
+        if (!g) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+        b->impl.reset(g);
+        g->n = n;
+        g->sc.assign(scenes, scenes + n);
+        g->P = P;
+        rc = g->create(tune);
+    } else {
+        auto* g = new (std::nothrow) mcg::GraphBatch<float>();
+        if (!g) return mcg::set_error(MCG_ENOMEM, "out of host memory");
+        b->impl.reset(g);
+        g->n = n;
+        g->sc.assign(scenes, scenes + n);
+        g->P = P;
+        rc = g->create(tune);
+    }
+    if (rc) return rc;
+    *out = b.release();
+    return MCG_OK;
+}
+
+int mcg_batch_run(mcg_batch* batch, const uint64_t* seeds, double* const* outs, const int64_t* out_lens,
+                  mcg_stats* stats) {
+    if (!batch || !batch->impl || !outs || !out_lens) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    for (int i = 0; i < batch->impl->n; ++i) {
+        if (!outs[i]) return mcg::set_error(MCG_EINVAL, "NULL output");
+        if (out_lens[i] < batch->impl->need(i)) return mcg::set_error(MCG_ERANGE, "output buffer too small");
+    }
+    return batch->impl->run(seeds, outs, stats);
+}
+
+int mcg_batch_destroy(mcg_batch* batch) {
+    delete batch;
     return MCG_OK;
 }
 

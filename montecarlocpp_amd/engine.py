@@ -21,7 +21,8 @@ EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_pro
            "mcg_server_create", "mcg_server_submit", "mcg_server_step", "mcg_server_completed",
            "mcg_server_poll",
            "mcg_server_cancel", "mcg_server_info", "mcg_server_destroy",
-           "mcg_solve", "mcg_trace", "mcg_batch_solve")
+           "mcg_solve", "mcg_trace", "mcg_batch_solve", "mcg_batch_create", "mcg_batch_run",
+           "mcg_batch_destroy")
 
 TRACE_WIDTH = 16
 
@@ -80,6 +81,11 @@ def lib():
         L.mcg_batch_solve.argtypes = [C.c_int32, C.POINTER(C.c_void_p), PS,
                                       C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
                                       C.POINTER(abi.Stats), C.POINTER(abi.Tune)]
+        L.mcg_batch_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p), PS, C.POINTER(abi.Tune),
+                                       C.POINTER(C.c_void_p)]
+        L.mcg_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.POINTER(C.c_double)),
+                                    C.POINTER(C.c_int64), C.POINTER(abi.Stats)]
+        L.mcg_batch_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -216,6 +222,51 @@ def batch_solve(scenes_, probs, tune=None):
     check(lib().mcg_batch_solve(n, handles, parr, optr, lens, stats, C.byref(t)))
     fields = [o.reshape(i.cols, i.rows).T.copy() for o, i in zip(outs, infos)]
     return fields, [s.as_dict() for s in stats]
+
+
+class Batch:
+    """mcg_batch_*: a batch captured once into a HIP graph, replayed by run()."""
+
+    def __init__(self, scenes_, probs, tune=None):
+        n = len(scenes_)
+        self._scenes = list(scenes_)           # the scenes must outlive the batch
+        self.infos = [sc.plan(p) for sc, p in zip(scenes_, probs)]
+        self.outs = [np.zeros(i.rows * i.cols) for i in self.infos]
+        self._optr = (C.POINTER(C.c_double) * n)(*[abi.dptr(o) for o in self.outs])
+        self._lens = (C.c_int64 * n)(*[o.size for o in self.outs])
+        self.stats = (abi.Stats * n)()
+        handles = (C.c_void_p * n)(*[sc.handle.value for sc in scenes_])
+        parr = (abi.ProblemSpec * n)(*probs)
+        h = C.c_void_p()
+        check(lib().mcg_batch_create(n, handles, parr, C.byref(tune) if tune is not None else None,
+                                     C.byref(h)))
+        self.handle = h
+        self.n = n
+
+    def run(self, seeds=None):
+        """Replay the graph (optionally with new per-solve seeds); returns
+        (list of fields, list of stats dicts) as batch_solve does."""
+        sp = (C.c_uint64 * self.n)(*[int(x) for x in seeds]) if seeds is not None else None
+        check(lib().mcg_batch_run(self.handle, sp, self._optr, self._lens, self.stats))
+        fields = [o.reshape(i.cols, i.rows).T.copy() for o, i in zip(self.outs, self.infos)]
+        return fields, [s.as_dict() for s in self.stats]
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().mcg_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def autotune_batch(scenes_, probs, sample_fraction=0.1):

@@ -36,3 +36,36 @@ def test_batch_equals_individual_solves(precision, tune):
         assert np.all(err < 1e-10), (k, err)
     for s in sc:
         s.close()
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_FLIGHT])
+def test_graph_batch_replays_equal_eager(precision):
+    """A batch captured into a HIP graph gives, on every replay, what the
+    eager mcg_batch_solve gives, with eager work and synchronisations between
+    the replays; new seeds on a replay equal an eager batch with those seeds."""
+    cases = workloads.slab_batch(10, precision=precision)
+    cases = [(m, d, scenes.problem(min(p.nemit, 40_000), seed=p.seed, precision=precision))
+             for m, d, p in cases]
+    cases.append((materials.si_like(300.0), scenes.porous_film(div=(16, 16, 0)),
+                  scenes.problem(20_000, maxscat=30, seed=77, precision=precision)))
+    sc = [engine.Scene(m, d) for m, d, _ in cases]
+    probs = [p for _, _, p in cases]
+    fe, se = engine.batch_solve(sc, probs)
+    with engine.Batch(sc, probs) as b:
+        for rep in range(3):
+            fg, sg = b.run()
+            print("replay", rep, flush=True)
+            for k in range(len(probs)):
+                assert sg[k]["steps"] == se[k]["steps"], (rep, k)
+                assert np.all(rownorm_rel(fg[k], fe[k]) < 1e-10), (rep, k)
+            sc[0].solve(probs[0])          # eager work between replays
+        seeds = [1000 + k for k in range(len(probs))]
+        fg, sg = b.run(seeds)
+        for p, s in zip(probs, seeds):
+            p.seed = s
+        fe2, se2 = engine.batch_solve(sc, probs)
+        for k in range(len(probs)):
+            assert sg[k]["steps"] == se2[k]["steps"]
+            assert np.all(rownorm_rel(fg[k], fe2[k]) < 1e-10)
+    for s in sc:
+        s.close()
