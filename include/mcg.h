@@ -240,7 +240,9 @@ int mcg_scene_plan(const mcg_scene* scene, const mcg_problem_spec* prob, mcg_pro
 
 /* FieldProblem::solve (problem.cpp:370-445) for the whole bundle set:
  * out receives rows x cols doubles, column-major, equal to the reference's
- * return value power_ * postProc(data) / cellVol.  counts may be NULL. */
+ * return value power_ * postProc(data) / cellVol.  counts may be NULL.
+ * The scene keeps its device workspace between solves: one host thread
+ * solves (or traces, or autotunes) a given scene at a time. */
 int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob,
               double* out, int64_t out_len, mcg_counts* counts, mcg_stats* stats);
 

@@ -1048,6 +1048,11 @@ struct DevBuf {
         if (e != hipSuccess) { p = nullptr; return set_error(MCG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
         return MCG_OK;
     }
+    // at least `count` elements, reusing the allocation when it is large enough
+    int reserve(size_t count) {
+        if (p && count <= n) return MCG_OK;
+        return alloc(count);
+    }
     int upload(const T* src, size_t count) {
         int rc = alloc(count);
         if (rc || count == 0) return rc;
@@ -1142,6 +1147,17 @@ struct mcg_scene {
     int num_cu = 0;
     size_t lds_limit = 0;
     mcg_tune tune{0, 0, 0, 0, 0};   // launch shape for this scene's solves (0 = default)
+    // per-solve device workspace, kept between solves (one thread solves a
+    // scene at a time): tallies (direct then difference array), counters,
+    // emitter CDF, timing events
+    mcg::DevBuf<double> ws_tally;
+    mcg::DevBuf<unsigned long long> ws_ctr;
+    mcg::DevBuf<int64_t> ws_cdf;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~mcg_scene() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
 };
 
 namespace mcg {
@@ -1217,6 +1233,18 @@ void fill_params(mcg_scene* sc, const HostProblem& p, KParams<F>& K) {
     K.trace_id = -1;
 }
 
+// Clears tallies and counters: one launch instead of several memsets, and
+// usable inside a captured graph (memset nodes in captured graphs are
+// avoided).
+__global__ void zero_kernel(double* a, size_t na, double* b, size_t nb, unsigned long long* c, size_t nc) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
+        if (i < na) a[i] = 0.0;
+        else if (i < na + nb) b[i - na] = 0.0;
+        else c[i - na - nb] = 0ull;
+    }
+}
+
 template <typename F>
 int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* counts, mcg_stats* stats,
               int64_t trace_id = -1, double* trace = nullptr, int64_t trace_cap = 0) {
@@ -1227,28 +1255,29 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     const int cols = int(h.cols);
     if (p.rows > (1 << 20) || h.cols * p.rows > (1ll << 31)) return set_error(MCG_ERANGE, "tally too large");
     if (p.maxloop <= 0) return set_error(MCG_EINVAL, "maxloop must be positive");
-    DevBuf<double> direct, diff;
-    DevBuf<unsigned long long> ctr;
-    DevBuf<int64_t> cdf;
+    const size_t nd = size_t(rows) * cols, nf = size_t(rows) * (cols + 1);
     DevBuf<uint32_t> cnt;
     int rc;
-    if ((rc = direct.alloc(size_t(rows) * cols))) return rc;
-    if ((rc = diff.alloc(size_t(rows) * (cols + 1)))) return rc;
-    if ((rc = ctr.alloc(C_COUNT))) return rc;
-    if ((rc = cdf.upload(p.emit_cdf.data(), p.emit_cdf.size()))) return rc;
+    if ((rc = sc->ws_tally.reserve(nd + nf))) return rc;
+    if ((rc = sc->ws_ctr.reserve(C_COUNT))) return rc;
+    if ((rc = sc->ws_cdf.reserve(p.emit_cdf.size()))) return rc;
+    HIP_TRY(hipMemcpy(sc->ws_cdf.p, p.emit_cdf.data(), p.emit_cdf.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    double* const direct = sc->ws_tally.p;
+    double* const diff = direct + nd;
+    unsigned long long* const ctr = sc->ws_ctr.p;
     const bool want_counts = counts && counts->nscat && counts->nbdry && counts->nstep && counts->fate;
     if (want_counts && (rc = cnt.alloc(size_t(4) * p.count))) return rc;
-    HIP_TRY(hipMemset(direct.p, 0, direct.n * sizeof(double)));
-    HIP_TRY(hipMemset(diff.p, 0, diff.n * sizeof(double)));
-    HIP_TRY(hipMemset(ctr.p, 0, ctr.n * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(zero_kernel, dim3(int(std::min<size_t>(1024, (nd + nf + 255) / 256 + 1))), dim3(256), 0, 0,
+                       direct, nd + nf, static_cast<double*>(nullptr), size_t(0), ctr, size_t(C_COUNT));
+    HIP_TRY(hipGetLastError());
 
     KParams<F> K{};
     fill_params<F>(sc, p, K);
-    K.emit_cdf = cdf.p;
-    K.direct = direct.p;
-    K.diff = diff.p;
-    K.counters = ctr.p;
-    K.next = ctr.p + C_NEXT;
+    K.emit_cdf = sc->ws_cdf.p;
+    K.direct = direct;
+    K.diff = diff;
+    K.counters = ctr;
+    K.next = ctr + C_NEXT;
     if (want_counts) {
         K.c_nscat = cnt.p;
         K.c_nbdry = cnt.p + p.count;
@@ -1280,25 +1309,22 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     if (sc->tune.blocks_per_cu > 0) blocks_per_cu = std::min(blocks_per_cu, int(sc->tune.blocks_per_cu));
     const int64_t needed = (p.count + block - 1) / block;
     const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, 0));
+    if (!sc->ev0) HIP_TRY(hipEventCreate(&sc->ev0));
+    if (!sc->ev1) HIP_TRY(hipEventCreate(&sc->ev1));
+    HIP_TRY(hipEventRecord(sc->ev0, 0));
     void* args[] = {&K};
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, 0));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(e1, 0));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(sc->ev1, 0));
+    HIP_TRY(hipEventSynchronize(sc->ev1));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(hipEventElapsedTime(&ms, sc->ev0, sc->ev1));
 
-    std::vector<double> hd(direct.n), hf(diff.n);
+    std::vector<double> hd(nd), hf(nf);
     unsigned long long hc[C_COUNT];
-    HIP_TRY(hipMemcpy(hd.data(), direct.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hf.data(), diff.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hd.data(), direct, nd * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hf.data(), diff, nf * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
     if (want_counts) {
         HIP_TRY(hipMemcpy(counts->nscat, cnt.p, p.count * 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(counts->nbdry, cnt.p + p.count, p.count * 4, hipMemcpyDeviceToHost));
@@ -1512,17 +1538,6 @@ size_t scene_bytes(const HostScene& h) {
 
 // ================================================================ graph batch
 namespace mcg {
-
-// Clears a batch's tallies and counters inside its graph (a kernel rather
-// than hipMemsetAsync: memset nodes in captured graphs are avoided).
-__global__ void zero_kernel(double* a, size_t na, double* b, size_t nb, unsigned long long* c, size_t nc) {
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
-        if (i < na) a[i] = 0.0;
-        else if (i < na + nb) b[i - na] = 0.0;
-        else c[i - na - nb] = 0ull;
-    }
-}
 
 struct GraphBatchBase {
     virtual ~GraphBatchBase() = default;
