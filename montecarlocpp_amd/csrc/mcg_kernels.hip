@@ -237,10 +237,11 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
 
 // A 2-D / 3-D tally walk in progress: each axis's next crossing parameter
 // pk (2 once the axis has none left), the plane index of that crossing and
-// of the axis's last one, the column step of each axis, the current column,
-// the parameter reached and the step's amounts (exact in F: a sign times an
-// F value).  q is the segment's coordinate change (FP64) or its reciprocal
-// (FP32), see crossing_param.
+// of the axis's last one (FP32: nxt counts the crossings left and last is
+// unused), the column step of each axis, the current column, the parameter
+// reached and the step's amounts (exact in F: a sign times an F value).  q is
+// the segment's coordinate change (FP64, see crossing_param) or the
+// parameter step |1/dc| between an axis's crossings (FP32).
 template <typename F>
 struct Walk {
     F bc[3], q[3], pk[3];
@@ -290,7 +291,10 @@ __device__ __forceinline__ bool walk_next(Walk<F>& W, const Sink& sink) {
     for (int k = 0; k < 3; ++k) {
         if (W.pk[k] != pmin) continue;
         W.col += W.dcol[k];
-        if (W.nxt[k] == W.last[k]) {
+        if constexpr (sizeof(F) == 4) {
+            W.pk[k] = W.nxt[k] == 0 ? F(2) : W.pk[k] + W.q[k];
+            W.nxt[k] -= W.nxt[k] > 0 ? 1 : 0;
+        } else if (W.nxt[k] == W.last[k]) {
             W.pk[k] = F(2);
         } else {
             W.nxt[k] += W.dcol[k] > 0 ? 1 : -1;
@@ -405,6 +409,12 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         }
         W.q[k] = sizeof(F) == 8 ? dc : F(1) / dc;
         W.pk[k] = crossing_param<F>(W.nxt[k], bc[k], W.q[k]);
+        if constexpr (sizeof(F) == 4) {
+            // FP32 flight steps the parameter by |1/dc| per crossing and
+            // counts the crossings left in nxt
+            W.nxt[k] = W.last[k] > W.nxt[k] ? W.last[k] - W.nxt[k] : W.nxt[k] - W.last[k];
+            W.q[k] = fabsf(W.q[k]);
+        }
     }
     W.prev = F(0);
     W.base = r0 * sink.cols;
