@@ -528,6 +528,17 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
         P.kind = ps.kind;
         P.spec_thr = ps.kind == MCG_PARTIAL ? prob_to_thr(ps.specularity)
                                             : (ps.kind == MCG_SPEC ? 0xFFFFFFFFu : 0u);
+        // the pore runs through the cell's faces across its axis: a face
+        // there keeps the hole (Peri back onto this cell) or closes it; an
+        // Inter face, or a Peri face onto another cell, would let bundles in
+        // through the hole
+        for (int b = s.sdoms[ps.sdom].pbeg; b < s.sdoms[ps.sdom].pend; ++b) {
+            const DevPlane<double>& F = s.planes[b];
+            if (F.n[2] == 0.0) continue;
+            if (F.kind == MCG_INTER) return set_error(MCG_EINVAL, "a pore cannot open onto an Inter face");
+            if (F.kind == MCG_PERI && F.target != ps.sdom)
+                return set_error(MCG_EINVAL, "a pore's Peri faces must pair within its cell");
+        }
         s.sdoms[ps.sdom].pore = int32_t(s.pores.size());
         s.pores.push_back(P);
     }
@@ -574,7 +585,11 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             e.sdom = si;
             e.sign = temp_now[pi] >= 0.0 ? 1 : 0;
             s.emitters.push_back(e);
-            s.emit_weight.push_back(face_area[pi] * std::fabs(temp_now[pi]));
+            // a face across the pore's axis has its disk cut out
+            const double area = pk >= 0 && P.n[2] != 0.0
+                                    ? face_area[pi] - 3.141592653589793 * (s.pores[pk].r * s.pores[pk].r)
+                                    : face_area[pi];
+            s.emit_weight.push_back(area * std::fabs(temp_now[pi]));
         }
     }
     // tally columns and per-column cell volumes

@@ -189,6 +189,18 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
         }
         L.pos = dev::V3<F>{E.o[0] + (a * E.si[0] + b * E.sj[0]), E.o[1] + (a * E.si[1] + b * E.sj[1]),
                            E.o[2] + (a * E.si[2] + b * E.sj[2])};
+        const int pk = K.npore > 0 ? K.sdoms[E.sdom].pore : -1;
+        if (pk >= 0) {   // a wall across the pore (a box face): redraw inside its disk
+            const DevPore<F>& Q = K.pores[pk];
+            for (uint32_t k = 0; k < kMaxPosRedraw; ++k) {
+                const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
+                if (!(dx * dx + dy * dy < Q.r2)) break;
+                const uint4 ba = ev.block(kPosRedrawBlock + 2u * k);
+                const F s1 = dev::Real<F>::uniform(ba.x, ba.y), s2 = dev::Real<F>::uniform(ba.z, ba.w);
+                L.pos = dev::V3<F>{E.o[0] + (s1 * E.si[0] + s2 * E.sj[0]), E.o[1] + (s1 * E.si[1] + s2 * E.sj[1]),
+                                   E.o[2] + (s1 * E.si[2] + s2 * E.sj[2])};
+            }
+        }
         d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, false));
         L.sign = E.sign ? 1 : -1;
     } else {
@@ -694,7 +706,9 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             const F cq = dx * dx + dy * dy - Q.r2;
             const F disc = bq * bq - aq * cq;
             if (disc >= F(0)) {
-                const F t = dev::Real<F>::div(cq, dev::Real<F>::sqrt_(disc) - bq);
+                // a start on (or, by rounding, just inside) the wall heading
+                // in meets it at once
+                const F t = cq <= F(0) ? F(0) : dev::Real<F>::div(cq, dev::Real<F>::sqrt_(disc) - bq);
                 if (t < dmin) {
                     dmin = t;
                     hit = -3;

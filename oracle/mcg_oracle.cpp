@@ -753,6 +753,17 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
                 return fail(MCG_EINVAL, "specularity outside [0, 1]");
             P.thr = ps.specularity >= 1.0 ? 0xFFFFFFFFu : uint32_t(ps.specularity * 4294967296.0);
         }
+        // the pore runs through the cell's faces across its axis: a face
+        // there keeps the hole (Peri back onto this cell) or closes it; an
+        // Inter face, or a Peri face onto another cell, would let bundles in
+        // through the hole
+        for (int b = S.pbeg; b < S.pend; ++b) {
+            const Plane& F = D.planes[b];
+            if (F.n.z == 0.0) continue;
+            if (F.kind == MCG_INTER) return fail(MCG_EINVAL, "a pore cannot open onto an Inter face");
+            if (F.kind == MCG_PERI && D.planes[F.pair].sdom != ps.sdom)
+                return fail(MCG_EINVAL, "a pore's Peri faces must pair within its cell");
+        }
         S.pore = int(D.pores.size());
         D.pores.push_back(P);
     }
@@ -769,7 +780,9 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
         if (wv != 0.0) D.emitters.push_back({true, si, wv});
         for (int pi : faceEmit[si]) {
             const Plane& P = D.planes[pi];
-            D.emitters.push_back({false, pi, P.area * std::fabs(P.T)});
+            double area = P.area;   // a face across the pore's axis has its disk cut out
+            if (S.pore >= 0 && P.n.z != 0.0) area -= 3.141592653589793 * (D.pores[S.pore].r * D.pores[S.pore].r);
+            D.emitters.push_back({false, pi, area * std::fabs(P.T)});
         }
     }
     // field columns (field.cpp:25-45) and cell volumes (field.cpp:53-77)
@@ -926,6 +939,15 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
         else
             local = add(scale(1.0 - r1, P.si), scale(1.0 - r2, P.sj));
         pos = add(P.o, local);
+        if (D.sdoms[P.sdom].pore >= 0) {   // a wall across the pore: redraw inside its disk
+            const Pore& Q = D.pores[D.sdoms[P.sdom].pore];
+            for (int k = 0; k < kMaxPosRedraw; ++k) {
+                const double dx = pos.x - Q.cx, dy = pos.y - Q.cy;
+                if (!(dx * dx + dy * dy < Q.r2)) break;
+                const int w = 4 * (B_POSRE + 2 * k);
+                pos = add(P.o, add(scale(g.u53(w), P.si), scale(g.u53(w + 2), P.sj)));
+            }
+        }
         dir = mul(P.rot, draw_aniso(g, false));
         sign = P.T >= 0.0;
         sdom = P.sdom;
@@ -1014,7 +1036,9 @@ int advect(const Domain& D, const Sdom& S, Phonon& ph, double& dist) {
             const double cq = dx * dx + dy * dy - Q.r2;
             const double disc = bq * bq - aq * cq;
             if (disc >= 0.0) {
-                const double t = cq / (std::sqrt(disc) - bq);
+                // a start on (or, by rounding, just inside) the wall heading
+                // in meets it at once
+                const double t = cq <= 0.0 ? 0.0 : cq / (std::sqrt(disc) - bq);
                 if (t < minDistance) {
                     minDistance = t;
                     hit = -3;

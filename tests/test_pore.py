@@ -143,6 +143,64 @@ def test_pore_errors(lib, mat):
     s = d.add_box((0, 0, 0), DIM, (4, 4, 0), walls)
     d.add_pore(s, C, 5e-8, kind=abi.PARTIAL, specularity=1.5)
     assert "specularity" in _err(lib, d, mat)
+    # the hole through a face across the pore's axis must not open onto
+    # another cell: Inter there, or Peri onto a different cell
+    lo = list(walls)
+    lo[scenes.TOP] = f(abi.INTER)
+    hi = list(walls)
+    hi[scenes.BOT] = f(abi.INTER)
+    hi[scenes.BAC] = f(abi.ISOT, 1.0)
+    d = scenes.Domain()
+    a = d.add_box((0, 0, 0), DIM, (4, 4, 0), lo)
+    b = d.add_box((0, 0, DIM[2]), DIM, (4, 4, 0), hi)
+    d.pair(a, scenes.TOP, b, scenes.BOT)
+    d.add_pore(a, C, 5e-8)
+    assert "Inter" in _err(lib, d, mat)
+    lo[scenes.TOP] = f(abi.PERI)
+    hi[scenes.BOT] = f(abi.PERI)
+    d = scenes.Domain()
+    a = d.add_box((0, 0, 0), DIM, (4, 4, 0), lo)
+    b = d.add_box((0, 0, DIM[2]), DIM, (4, 4, 0), hi)
+    d.pair(a, scenes.TOP, b, scenes.BOT, transl=(0.0, 0.0, 0.0))
+    d.add_pore(a, C, 5e-8)
+    assert "within its cell" in _err(lib, d, mat)
+
+
+def _cross_plane(T=1.0):
+    """The pored box heated through its bottom face: isothermal walls across
+    the pore's axis, so the face emitters carry the hole."""
+    f = scenes.face
+    walls = [f(abi.DIFF)] * 6
+    walls[scenes.BOT] = f(abi.ISOT, T)
+    walls[scenes.TOP] = f(abi.ISOT, 0.0)
+    d = scenes.Domain("cross_plane")
+    s = d.add_box((0, 0, 0), DIM, (0, 0, 8), walls)
+    d.add_pore(s, C, R)
+    return d
+
+
+def test_face_emitter_weight_excludes_hole(mat):
+    d = _cross_plane(2.0)
+    for lib in (oracle, engine):
+        info, w, n = lib.plan(mat, d, scenes.problem(1000))
+        np.testing.assert_allclose(w, [(DIM[0] * DIM[1] - math.pi * R * R) * 2.0], rtol=1e-14)
+
+
+def test_face_emission_outside_pore(mat):
+    """Emission from a face across the pore is uniform over the face minus
+    the disk, and every bundle of the scene stays in the material."""
+    d = _cross_plane()
+    n = 40000
+    e = oracle.sample_emission(mat, d, scenes.problem(n, seed=7), n)
+    assert np.all(e[:, 2] == 0.0)
+    rr = np.hypot(e[:, 0] - C[0], e[:, 1] - C[1])
+    assert rr.min() >= R
+    area = DIM[0] * DIM[1] - math.pi * R * R
+    p_ann = math.pi * ((1.4 * R) ** 2 - R * R) / area
+    k = np.count_nonzero(rr < 1.4 * R)
+    assert abs(k - n * p_ann) < 5 * math.sqrt(n * p_ann * (1 - p_ann))
+    _, st = oracle.solve(mat, d, scenes.problem(5000, maxscat=50, seed=3))[:2]
+    assert st["escapes"] == 0 and st["bundles"] == 5000
 
 
 def test_emission_outside_pore_uniform(mat):
