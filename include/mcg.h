@@ -262,7 +262,7 @@ int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, do
  * Many independent solves in one persistent launch.  Each solve has its
  * own scene, problem and output; results equal those of mcg_solve. */
 struct mcg_tune {
-    int32_t block;             /* threads per block: 64 ... 1024 (0 = auto)    */
+    int32_t block;             /* threads per block: a multiple of 64 up to 1024 (0 = auto) */
     int32_t per_thread;        /* phonons carried per thread: 1, 2, 4, 8        */
     int32_t chunk;             /* bundles claimed per wavefront grab            */
     int32_t blocks_per_cu;     /* occupancy target (0 = auto)                   */

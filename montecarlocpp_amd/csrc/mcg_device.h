@@ -16,6 +16,14 @@ namespace mcg {
 namespace dev {
 
 // ------------------------------------------------------------------ Philox
+// a ^ b ^ k in one VALU op (gfx950 V_BITOP3_B32, truth table 0x96); the
+// round key k is wavefront-uniform and sits in an SGPR
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t k) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(k));
+    return r;
+}
+
 __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                         uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -24,7 +32,7 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
         const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
         const uint32_t lo0 = uint32_t(p0), hi0 = uint32_t(p0 >> 32);
         const uint32_t lo1 = uint32_t(p1), hi1 = uint32_t(p1 >> 32);
-        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        const uint32_t n0 = xor3(hi1, c1, k0), n2 = xor3(hi0, c3, k1);
         c0 = n0;
         c1 = lo1;
         c2 = n2;
@@ -61,7 +69,7 @@ struct Real;
 template <>
 struct Real<double> {
     static constexpr double kMin = DBL_MIN;
-    // 53-bit uniform on [0, 1) from the word pair (a, b)
+    // 53-bit uniform on [0, 1): 27 bits of the high word a, 26 of the low b
     __device__ static __forceinline__ double uniform(uint32_t a, uint32_t b) {
         return (double(a >> 5) * 67108864.0 + double(b >> 6)) * (1.0 / 9007199254740992.0);
     }
@@ -82,7 +90,8 @@ struct Real<double> {
 template <>
 struct Real<float> {
     static constexpr float kMin = FLT_MIN;
-    // 24-bit uniform on [0, 1) from the first word of the pair
+    // 24-bit uniform on [0, 1) from the high word alone (the FP64 uniform's
+    // leading bits, so both precisions take the same decisions)
     __device__ static __forceinline__ float uniform(uint32_t a, uint32_t) {
         return float(a >> 8) * 5.9604644775390625e-8f;
     }
@@ -153,12 +162,14 @@ __device__ __forceinline__ V3<F> normalized(const V3<F>& v) {
     return v;
 }
 
-// drawIso (random.cpp:16-27): cos(theta) ~ U[-1, 1), phi ~ pi U[-1, 1)
+// drawIso (random.cpp:16-27): cos(theta) ~ U[-1, 1), phi ~ pi U[-1, 1).
+// An event's uniforms take their high words from block 0 (h) and their low
+// words from block 1 (l), which FP32 never evaluates.
 template <typename F>
-__device__ __forceinline__ V3<F> draw_iso(const uint4& b0) {
-    const F cosT = Real<F>::uniform(b0.x, b0.y) * F(2) + F(-1);
+__device__ __forceinline__ V3<F> draw_iso(const uint4& h, const uint4& l) {
+    const F cosT = Real<F>::uniform(h.x, l.x) * F(2) + F(-1);
     const F sinT = Real<F>::sqrt_(F(1) - cosT * cosT);
-    const F r = Real<F>::uniform(b0.z, b0.w) * F(2) + F(-1);
+    const F r = Real<F>::uniform(h.y, l.y) * F(2) + F(-1);
     F s, c;
     Real<F>::sincos_pi(r, s, c);
     return V3<F>{sinT * c, sinT * s, cosT};
@@ -167,13 +178,13 @@ __device__ __forceinline__ V3<F> draw_iso(const uint4& b0) {
 // drawAniso (random.cpp:29-44): sin^2(theta) = |r|, r ~ U[-1, 1); with
 // bidir the sign of r picks the hemisphere
 template <typename F>
-__device__ __forceinline__ V3<F> draw_aniso(const uint4& b0, bool bidir) {
-    const F r = Real<F>::uniform(b0.x, b0.y) * F(2) + F(-1);
+__device__ __forceinline__ V3<F> draw_aniso(const uint4& h, const uint4& l, bool bidir) {
+    const F r = Real<F>::uniform(h.x, l.x) * F(2) + F(-1);
     const F sg = (bidir && r < F(0)) ? F(-1) : F(1);
     const F sin2 = r < F(0) ? -r : r;
     const F sinT = Real<F>::sqrt_(sin2);
     const F cosT = sg * Real<F>::sqrt_(F(1) - sin2);
-    const F q = Real<F>::uniform(b0.z, b0.w) * F(2) + F(-1);
+    const F q = Real<F>::uniform(h.y, l.y) * F(2) + F(-1);
     F s, c;
     Real<F>::sincos_pi(q, s, c);
     return V3<F>{sinT * c, sinT * s, cosT};

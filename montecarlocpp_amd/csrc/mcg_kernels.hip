@@ -152,11 +152,28 @@ __device__ __forceinline__ bool inside(const DevPlane<F>* planes, const DevSdom<
     return S.pore < 0 || outside_pore(pores[S.pore], p);
 }
 
+// Block j of an event when it holds only the low words of uniforms whose
+// high words are in block j - 1: FP64 evaluates it, FP32 (which reads high
+// words alone) does not.
+template <typename F>
+__device__ __forceinline__ uint4 low_block(const dev::Event& ev, uint32_t j) {
+    if constexpr (sizeof(F) == 8) return ev.block(j);
+    else return make_uint4(0u, 0u, 0u, 0u);
+}
+
+// The partial-specularity word (word 3 of block 1): FP64 has the block at
+// hand, FP32 evaluates it here, on a partial face only.
+template <typename F>
+__device__ __forceinline__ uint32_t spec_word(const dev::Event& ev, const uint4& b1) {
+    if constexpr (sizeof(F) == 8) return b1.w;
+    else return ev.block(1).w;
+}
+
 // Free path: l = v tau (-ln(1 - u)), redrawn while l < the smallest normal
 // (material.cpp:215-224).
 template <typename F>
-__device__ __forceinline__ F draw_path(const dev::Event& ev, const uint4& b1, F mfp) {
-    F d = mfp * dev::Real<F>::neglog1m(dev::Real<F>::uniform(b1.x, b1.y));
+__device__ __forceinline__ F draw_path(const dev::Event& ev, const uint4& h, const uint4& l, F mfp) {
+    F d = mfp * dev::Real<F>::neglog1m(dev::Real<F>::uniform(h.z, l.z));
     for (uint32_t k = 0; d < dev::Real<F>::kMin; ++k) {
         const uint4 b = ev.block(4u + k / 2u);
         const uint32_t a = (k & 1u) ? b.z : b.x, c = (k & 1u) ? b.w : b.y;
@@ -177,11 +194,11 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
     }
     const DevEmitter<F>& E = K.emitters[lo];
     const dev::Event ev{K.k0, K.k1, 0u, uint32_t(n), uint32_t(uint64_t(n) >> 32)};
-    const uint4 b0 = ev.block(0), b1 = ev.block(1), b2 = ev.block(2);
-    L.mode = dev::alias_draw(flux_alias, uint32_t(K.nmodes), b1.z);
+    const uint4 b0 = ev.block(0), b1 = low_block<F>(ev, 1), b2 = ev.block(2), b3 = low_block<F>(ev, 3);
+    L.mode = dev::alias_draw(flux_alias, uint32_t(K.nmodes), b0.w);
     dev::V3<F> d;
     if (!E.volume) {
-        const F r1 = dev::Real<F>::uniform(b2.x, b2.y), r2 = dev::Real<F>::uniform(b2.z, b2.w);
+        const F r1 = dev::Real<F>::uniform(b2.x, b3.x), r2 = dev::Real<F>::uniform(b2.y, b3.y);
         F a = r1, b = r2;
         if (E.shape == SHAPE_TRI && !(r1 + r2 < F(1))) {
             a = F(1) - r1;
@@ -195,18 +212,17 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
             for (uint32_t k = 0; k < kMaxPosRedraw; ++k) {
                 const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
                 if (!(dx * dx + dy * dy < Q.r2)) break;
-                const uint4 ba = ev.block(kPosRedrawBlock + 2u * k);
-                const F s1 = dev::Real<F>::uniform(ba.x, ba.y), s2 = dev::Real<F>::uniform(ba.z, ba.w);
+                const uint4 ba = ev.block(kPosRedrawBlock + 2u * k), bb = low_block<F>(ev, kPosRedrawBlock + 2u * k + 1u);
+                const F s1 = dev::Real<F>::uniform(ba.x, bb.x), s2 = dev::Real<F>::uniform(ba.y, bb.y);
                 L.pos = dev::V3<F>{E.o[0] + (s1 * E.si[0] + s2 * E.sj[0]), E.o[1] + (s1 * E.si[1] + s2 * E.sj[1]),
                                    E.o[2] + (s1 * E.si[2] + s2 * E.sj[2])};
             }
         }
-        d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, false));
+        d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, b1, false));
         L.sign = E.sign ? 1 : -1;
     } else {
-        const uint4 b3 = ev.block(3);
-        F c0 = dev::Real<F>::uniform(b2.x, b2.y), c1 = dev::Real<F>::uniform(b2.z, b2.w),
-          c2 = dev::Real<F>::uniform(b3.x, b3.y);
+        F c0 = dev::Real<F>::uniform(b2.x, b3.x), c1 = dev::Real<F>::uniform(b2.y, b3.y),
+          c2 = dev::Real<F>::uniform(b2.z, b3.z);
         if (E.shape != MCG_PARALLELEPIPED && c0 + c1 > F(1)) {
             c0 = F(1) - c0;
             c1 = F(1) - c1;
@@ -230,19 +246,19 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
             for (uint32_t k = 0; k < kMaxPosRedraw; ++k) {
                 const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
                 if (!(dx * dx + dy * dy < Q.r2)) break;
-                const uint4 ba = ev.block(kPosRedrawBlock + 2u * k), bb = ev.block(kPosRedrawBlock + 2u * k + 1u);
+                const uint4 ba = ev.block(kPosRedrawBlock + 2u * k), bb = low_block<F>(ev, kPosRedrawBlock + 2u * k + 1u);
                 const dev::V3<F> q = dev::matvec(
-                    E.mat, dev::V3<F>{dev::Real<F>::uniform(ba.x, ba.y), dev::Real<F>::uniform(ba.z, ba.w),
-                                      dev::Real<F>::uniform(bb.x, bb.y)});
+                    E.mat, dev::V3<F>{dev::Real<F>::uniform(ba.x, bb.x), dev::Real<F>::uniform(ba.y, bb.y),
+                                      dev::Real<F>::uniform(ba.z, bb.z)});
                 L.pos = dev::V3<F>{E.o[0] + q.x, E.o[1] + q.y, E.o[2] + q.z};
             }
         }
-        d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, true));
+        d = dev::matvec(E.rot, dev::draw_aniso<F>(b0, b1, true));
         L.sign = (d.x * E.grad[0] + d.y * E.grad[1] + d.z * E.grad[2] < F(0)) ? 1 : -1;
     }
     L.dir = dev::normalized(d);
     L.sdom = E.sdom;
-    L.scat_next = draw_path<F>(ev, b1, mfp_tab[L.mode]);
+    L.scat_next = draw_path<F>(ev, b0, b1, mfp_tab[L.mode]);
     L.nscat = L.nbdry = L.nstep = 0;
     L.id = n;
 }
@@ -443,11 +459,11 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
 // points away from the axis (into the material); the reflection and the
 // Lambertian rotation are built from it as for a planar face.
 template <typename F>
-__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const uint4& b0, const uint4& b1,
-                                             Lane<F>& L) {
+__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Event& ev, const uint4& b0,
+                                             const uint4& b1, Lane<F>& L) {
     const dev::V3<F> n = dev::normalized(dev::V3<F>{(L.pos.x - Q.cx) / Q.r, (L.pos.y - Q.cy) / Q.r, F(0)});
     int kind = Q.kind;
-    if (kind == MCG_PARTIAL) kind = b1.w < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
+    if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
     if (kind == MCG_SPEC) {
         const F nv[3] = {n.x, n.y, n.z};
         F r[9];
@@ -469,7 +485,7 @@ __device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const uint4& b
     const F r[9] = {F(1) - (tyy + tzz), txy + twz, txz - twy,
                     txy - twz, F(1) - (txx + tzz), tyz + twx,
                     txz + twy, tyz - twx, F(1) - (txx + tyy)};
-    L.dir = dev::normalized(dev::matvec(r, dev::draw_aniso<F>(b0, false)));
+    L.dir = dev::normalized(dev::matvec(r, dev::draw_aniso<F>(b0, b1, false)));
     L.nscat++;
 }
 
@@ -600,14 +616,14 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
     const bool intr = hit < 0;
     const DevPlane<F>& P = V.planes[intr ? S.pbeg : hit];
     int kind = intr ? -1 : P.kind;
-    if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+    if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < P.spec_thr ? MCG_SPEC : MCG_DIFF;
     const bool diff = kind == MCG_DIFF;
     // direction: isotropic cos(theta) = r, or Lambertian sin^2(theta) = |r|
-    const F r = dev::Real<F>::uniform(b0.x, b0.y) * F(2) + F(-1);
+    const F r = dev::Real<F>::uniform(b0.x, b1.x) * F(2) + F(-1);
     const F ar = r < F(0) ? -r : r;
     const F sinT = dev::Real<F>::sqrt_(intr ? F(1) - r * r : ar);
     const F cosT = intr ? r : dev::Real<F>::sqrt_(F(1) - ar);
-    const F q = dev::Real<F>::uniform(b0.z, b0.w) * F(2) + F(-1);
+    const F q = dev::Real<F>::uniform(b0.y, b1.y) * F(2) + F(-1);
     F sq, cq;
     dev::Real<F>::sincos_pi(q, sq, cq);
     const dev::V3<F> v{sinT * cq, sinT * sq, cosT};
@@ -618,8 +634,8 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
     L.nscat += (intr || diff) ? 1u : 0u;
     L.nbdry += intr ? 0u : 1u;
     if (intr) {
-        L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
-        L.scat_next = draw_path<F>(ev, b1, V.mfp[L.mode]);
+        L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b0.w);
+        L.scat_next = draw_path<F>(ev, b0, b1, V.mfp[L.mode]);
     } else if (kind == MCG_PERI) {
         const dev::V3<F> p = dev::matvec(P.prot, L.pos);
         L.pos = dev::V3<F>{p.x + P.pt[0], p.y + P.pt[1], p.z + P.pt[2]};
@@ -756,24 +772,24 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         walking = accumulate<F, kDefer>(S, sink, pre, L.pos, r0, nv, amt, W);
 
         // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
-        // the event's first two Philox blocks serve every branch below; they
-        // are drawn once, outside the branches, so that a wavefront whose
-        // lanes take different branches evaluates them once
+        // the event's Philox blocks serve every branch below; they are drawn
+        // once, outside the branches, so that a wavefront whose lanes take
+        // different branches evaluates them once (FP32: block 0 alone)
         const dev::Event ev{K.k0, K.k1, L.nstep, uint32_t(L.id), uint32_t(uint64_t(L.id) >> 32)};
-        const uint4 b0 = ev.block(0), b1 = ev.block(1);
+        const uint4 b0 = ev.block(0), b1 = low_block<F>(ev, 1);
         if constexpr (sizeof(F) == 4) {
             if (hit != -3) {
                 scatter_fp32<F>(K, V, S, ev, b0, b1, hit, L, fate);
             } else {
                 L.nbdry++;
-                pore_scatter<F>(K.pores[S.pore], b0, b1, L);
+                pore_scatter<F>(K.pores[S.pore], ev, b0, b1, L);
             }
         } else {
             if (hit == -1) {
-                L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b1.z);
-                L.dir = dev::normalized(dev::draw_iso<F>(b0));
+                L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b0.w);
+                L.dir = dev::normalized(dev::draw_iso<F>(b0, b1));
                 L.nscat++;
-                L.scat_next = draw_path<F>(ev, b1, V.mfp[L.mode]);
+                L.scat_next = draw_path<F>(ev, b0, b1, V.mfp[L.mode]);
             } else if (hit >= 0) {
                 L.nbdry++;
                 const DevPlane<F>& P = V.planes[hit];
@@ -784,7 +800,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                         L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
                         break;
                     case MCG_DIFF:   // boundary.cpp:308-312
-                        L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(b0, false)));
+                        L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(b0, b1, false)));
                         L.nscat++;
                         break;
                     case MCG_ISOT:   // boundary.cpp:456-460
@@ -817,7 +833,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                 }
             } else {
                 L.nbdry++;
-                pore_scatter<F>(K.pores[S.pore], b0, b1, L);
+                pore_scatter<F>(K.pores[S.pore], ev, b0, b1, L);
             }
         }
         if (fate == 0xFFFFFFFFu) {
@@ -1408,8 +1424,8 @@ int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
     // and at most 2048 threads per CU
     int per_cu = std::max<int>(1, int(s0->lds_limit / std::max<size_t>(B.lds, 1)));
     B.block = (tune && tune->block) ? tune->block : (per_cu == 1 ? 1024 : 512);
-    if (B.block != 64 && B.block != 128 && B.block != 256 && B.block != 512 && B.block != 1024)
-        return set_error(MCG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+    if (B.block < 64 || B.block > 1024 || B.block % 64 != 0)
+        return set_error(MCG_EINVAL, "block must be a multiple of 64 from 64 to 1024");
     per_cu = std::min(per_cu, std::max(1, 2048 / B.block));
     if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, int(tune->blocks_per_cu));
     B.grid = s0->num_cu * per_cu;
@@ -2033,9 +2049,8 @@ int mcg_scene_plan(const mcg_scene* scene, const mcg_problem_spec* prob, mcg_pro
 int mcg_scene_tune(mcg_scene* scene, const mcg_tune* tune) {
     if (!scene) return mcg::set_error(MCG_EINVAL, "NULL scene");
     mcg_tune t = tune ? *tune : mcg_tune{0, 0, 0, 0, 0};
-    if (t.block != 0 && t.block != 64 && t.block != 128 && t.block != 256 && t.block != 512 &&
-        t.block != 1024)
-        return mcg::set_error(MCG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+    if (t.block != 0 && (t.block < 64 || t.block > 1024 || t.block % 64 != 0))
+        return mcg::set_error(MCG_EINVAL, "block must be a multiple of 64 from 64 to 1024");
     if (t.chunk < 0 || t.blocks_per_cu < 0 || t.per_thread < 0)
         return mcg::set_error(MCG_EINVAL, "negative tuning value");
     if (t.chunk != 0 && t.chunk < 64) return mcg::set_error(MCG_EINVAL, "chunk must be at least 64");

@@ -16,17 +16,20 @@
 namespace mcg {
 
 // Word slots of one Philox event (event 0 = emission, event i + 1 = the
-// scatter after loop iteration i).  A double uniform uses the word pair
-// (w, w + 1); the FP32 flight path uses the first word of each pair.
+// scatter after loop iteration i).  A double uniform in slot w takes its high
+// word from w and its low word from w + 4, the same word of the next block;
+// the FP32 flight path uses the high word alone, so a scatter costs it one
+// Philox block (block 1 only for a partial-specularity test).  Free-path
+// redraws use adjacent pairs (W_REDRAW + 2k, + 2k + 1).
 enum : int {
     W_DIR1 = 0,
-    W_DIR2 = 2,
-    W_PATH = 4,
-    W_MODE = 6,
+    W_DIR2 = 1,
+    W_PATH = 2,
+    W_MODE = 3,
     W_SPEC = 7,
     W_POS1 = 8,
-    W_POS2 = 10,
-    W_POS3 = 12,
+    W_POS2 = 9,
+    W_POS3 = 10,
     W_REDRAW = 16
 };
 

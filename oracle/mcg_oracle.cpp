@@ -241,19 +241,20 @@ inline void philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out
 }
 
 // Word slots of one event (emission = event 0, the scatter that follows
-// loop iteration i = event i + 1).  A double uniform takes two words, the
-// integer draws one.
+// loop iteration i = event i + 1).  A double uniform in slot w takes its high
+// word from w and its low word from w + 4 (the same word of the next block);
+// the integer draws take one word; free-path redraws take adjacent pairs.
 enum : int {
     W_DIR1 = 0,     // direction: cos(theta) or sin^2(theta) draw
-    W_DIR2 = 2,     // direction: azimuth draw
-    W_PATH = 4,     // free path
-    W_MODE = 6,     // (w, p) alias draw
+    W_DIR2 = 1,     // direction: azimuth draw
+    W_PATH = 2,     // free path
+    W_MODE = 3,     // (w, p) alias draw
     W_SPEC = 7,     // partial specularity test
     W_POS1 = 8,     // emission position draws
-    W_POS2 = 10,
-    W_POS3 = 12,
-    W_REDRAW = 16,  // free path redraws, two words each
-    B_POSRE = 64    // emission position redraw k: blocks 64 + 2k and 65 + 2k
+    W_POS2 = 9,
+    W_POS3 = 10,
+    W_REDRAW = 16,  // free path redraw k: words 16 + 2k (high), 17 + 2k (low)
+    B_POSRE = 64    // emission position redraw k: blocks 64 + 2k (high) and 65 + 2k (low)
 };
 // Cap on emission position redraws (a pore covers at most pi/4 of its cell's
 // cross-section, so the cap is reached with probability below 1e-400).
@@ -261,26 +262,30 @@ constexpr int kMaxPosRedraw = 4096;
 
 struct EventRng {
     uint32_t k0, k1, s, nlo, nhi;
-    int cached = -1;
-    uint32_t buf[4];
+    int cached[2] = {-1, -1};
+    int next = 0;
+    uint32_t buf[2][4];
     EventRng(uint64_t seed, uint32_t event, uint64_t bundle)
         : k0(uint32_t(seed)), k1(uint32_t(seed >> 32)), s(event),
           nlo(uint32_t(bundle)), nhi(uint32_t(bundle >> 32)) {}
     uint32_t word(int w) {
-        int j = w >> 2;
-        if (j != cached) {
-            uint32_t ctr[4] = {uint32_t(j), s, nlo, nhi};
-            philox(ctr, k0, k1, buf);
-            cached = j;
-        }
-        return buf[w & 3];
+        const int j = w >> 2;
+        for (int i = 0; i < 2; ++i)
+            if (cached[i] == j) return buf[i][w & 3];
+        const int i = next;   // two cached blocks, replaced in turn
+        next ^= 1;
+        uint32_t ctr[4] = {uint32_t(j), s, nlo, nhi};
+        philox(ctr, k0, k1, buf[i]);
+        cached[i] = j;
+        return buf[i][w & 3];
     }
-    // 53-bit uniform on [0, 1) from words (w, w+1)
-    double u53(int w) {
-        uint32_t a = word(w) >> 5;
-        uint32_t b = word(w + 1) >> 6;
+    // 53-bit uniform on [0, 1): 27 bits of word hi, 26 of word lo
+    double u53_of(int hi, int lo) {
+        uint32_t a = word(hi) >> 5;
+        uint32_t b = word(lo) >> 6;
         return (double(a) * 67108864.0 + double(b)) * (1.0 / 9007199254740992.0);
     }
+    double u53(int w) { return u53_of(w, w + 4); }
     // Boost uniform_real_distribution(-1, 1): u * (max - min) + min
     double upm1(int w) { return u53(w) * 2.0 + -1.0; }
 };
@@ -919,7 +924,7 @@ void draw_scat_next(const Material& M, Phonon& ph, EventRng& g) {
     double mfp = M.modeMfp[ph.mode];
     double distance = mfp * -mcg_math::log(1.0 - g.u53(W_PATH));
     for (int k = 0; distance < DBL_MIN; ++k)
-        distance = mfp * -mcg_math::log(1.0 - g.u53(W_REDRAW + 2 * k));
+        distance = mfp * -mcg_math::log(1.0 - g.u53_of(W_REDRAW + 2 * k, W_REDRAW + 2 * k + 1));
     ph.scatNext = distance;
 }
 
@@ -945,7 +950,7 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
                 const double dx = pos.x - Q.cx, dy = pos.y - Q.cy;
                 if (!(dx * dx + dy * dy < Q.r2)) break;
                 const int w = 4 * (B_POSRE + 2 * k);
-                pos = add(P.o, add(scale(g.u53(w), P.si), scale(g.u53(w + 2), P.sj)));
+                pos = add(P.o, add(scale(g.u53(w), P.si), scale(g.u53(w + 1), P.sj)));
             }
         }
         dir = mul(P.rot, draw_aniso(g, false));
@@ -979,8 +984,8 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
                 if (!(dx * dx + dy * dy < P.r2)) break;
                 const int w = 4 * (B_POSRE + 2 * k);
                 c[0] = g.u53(w);
-                c[1] = g.u53(w + 2);
-                c[2] = g.u53(w + 4);
+                c[1] = g.u53(w + 1);
+                c[2] = g.u53(w + 2);
                 pos = add(S.o, mul(S.mat, v3(c[0], c[1], c[2])));
             }
         }
