@@ -430,6 +430,32 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             peri_pair.push_back(-1);
         }
         d.pend = int32_t(s.planes.size());
+        // six faces whose normals are coordinate axes: a box, advected axis by
+        // axis in FP32 flight (a face with inward normal +e_a bounds p_a from
+        // below at -c, one with -e_a from above at c)
+        d.box = 0;
+        if (nf == 6) {
+            int lo = 0, hi = 0;
+            for (int f = 0; f < 6; ++f) {
+                const DevPlane<double>& P = s.planes[d.pbeg + f];
+                int ax = -1, zeros = 0;
+                for (int k = 0; k < 3; ++k) {
+                    if (P.n[k] == 0.0) ++zeros;
+                    else if (P.n[k] == 1.0 || P.n[k] == -1.0) ax = k;
+                }
+                if (zeros != 2 || ax < 0) break;
+                if (P.n[ax] > 0.0) {
+                    d.bmin[ax] = -P.c;
+                    d.fmin[ax] = f;
+                    lo |= 1 << ax;
+                } else {
+                    d.bmax[ax] = P.c;
+                    d.fmax[ax] = f;
+                    hi |= 1 << ax;
+                }
+            }
+            d.box = lo == 7 && hi == 7 ? 1 : 0;
+        }
         s.sdoms.push_back(d);
     }
     std::vector<double> temp_now(face_temp);

@@ -63,6 +63,12 @@ constexpr uint32_t kMaxPosRedraw = 4096;
 #define MCG_EMIT_QUORUM 8
 #endif
 constexpr int kEmitQuorum = MCG_EMIT_QUORUM;
+// FP32 single solve: empty next-bundle slots a wavefront collects before it
+// emits into them
+#ifndef MCG_SLOT_QUORUM
+#define MCG_SLOT_QUORUM 16
+#endif
+constexpr int kSlotQuorum = MCG_SLOT_QUORUM;
 
 #ifndef MCG_WALK_QUORUM
 #define MCG_WALK_QUORUM 12
@@ -499,6 +505,7 @@ template <typename F>
 struct View {
     const DevPlane<F>* planes;
     const DevSdom<F>* sdoms;
+    const DevPore<F>* pores;
     const uint64_t* scat;
     const F* vel;
     const F* mfp;
@@ -506,10 +513,10 @@ struct View {
     double* diff;
 };
 
-// Stage one solve's tables, planes and subdomains in LDS and clear the LDS
-// tally; the caller synchronises.  Layout: [scat alias | vel | mfp] (when
-// kLdsTables), planes, subdomains, tally direct, tally diff (only when the
-// scene has 1-D tallies, the only users of range adds).
+// Stage one solve's tables, planes, subdomains and pores in LDS and clear
+// the LDS tally; the caller synchronises.  Layout: [scat alias | vel | mfp]
+// (when kLdsTables), planes, subdomains, pores, tally direct, tally diff
+// (only when the scene has 1-D tallies, the only users of range adds).
 template <typename F, bool kLdsTables, bool kLdsTally>
 __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int nthreads) {
     const int ntab = kLdsTables ? K.nmodes : 0;
@@ -519,6 +526,8 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
     off = (off + sizeof(DevPlane<F>) * K.nplanes + 15) & ~size_t(15);
     DevSdom<F>* s_sdoms = reinterpret_cast<DevSdom<F>*>(lds + off);
     off = (off + sizeof(DevSdom<F>) * K.nsdom + 15) & ~size_t(15);
+    DevPore<F>* s_pores = reinterpret_cast<DevPore<F>*>(lds + off);
+    off = (off + sizeof(DevPore<F>) * K.npore + 15) & ~size_t(15);
     double* s_direct = reinterpret_cast<double*>(lds + off);
     double* s_diff = s_direct + size_t(K.rows) * K.cols;
     if (kLdsTables) {
@@ -546,6 +555,10 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
     dst = reinterpret_cast<uint32_t*>(s_sdoms);
     const int ns = int(sizeof(DevSdom<F>) / 4) * K.nsdom;
     for (int i = tid; i < ns; i += nthreads) dst[i] = src[i];
+    src = reinterpret_cast<const uint32_t*>(K.pores);
+    dst = reinterpret_cast<uint32_t*>(s_pores);
+    const int nq = int(sizeof(DevPore<F>) / 4) * K.npore;
+    for (int i = tid; i < nq; i += nthreads) dst[i] = src[i];
     if (kLdsTally) {
         const int n = K.rows * K.cols + (K.lds_diff ? K.rows * (K.cols + 1) : 0);
         for (int i = tid; i < n; i += nthreads) s_direct[i] = 0.0;
@@ -560,6 +573,7 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
     }
     v.planes = s_planes;
     v.sdoms = s_sdoms;
+    v.pores = s_pores;
     return v;
 }
 
@@ -649,7 +663,7 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
         } else {
             for (int c = 0; c < P.icnt; ++c) {
                 const int cand = K.inter_list[P.ibeg + c];
-                if (inside(V.planes, V.sdoms[cand], K.pores, L.pos)) {
+                if (inside(V.planes, V.sdoms[cand], V.pores, L.pos)) {
                     target = cand;
                     break;
                 }
@@ -678,25 +692,40 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     // test below (at most kMaxFaces faces per cell, unrolled into registers)
     F dist[kMaxFaces], rate[kMaxFaces];
     if constexpr (sizeof(F) == 4) {
-        // every lane evaluates every face: a reciprocal is cheaper than the
-        // divergence of skipping the faces it moves away from
+        if (S.box) {
+            // an axis-aligned box: on each axis only the face ahead can be
+            // met, at (bound - p_a) / u_a
+            const F p[3] = {L.pos.x, L.pos.y, L.pos.z}, u[3] = {L.dir.x, L.dir.y, L.dir.z};
 #pragma unroll
-        for (int i = 0; i < kMaxFaces; ++i) {
-            dist[i] = F(1);
-            rate[i] = F(0);
-            const int b = S.pbeg + i;
-            if (b < S.pend) {
-                const DevPlane<F>& P = V.planes[b];
-                const F nd = dev::dot3(P.n, L.dir);
-                const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
-                                                                            __builtin_fmaf(P.n[0], L.pos.x, P.c)));
-                const F t = -d * __builtin_amdgcn_rcpf(nd);
-                if (nd < F(0) && t < dmin) {
+            for (int a = 0; a < 3; ++a) {
+                const bool up = u[a] > F(0);
+                const F t = ((up ? S.bmax[a] : S.bmin[a]) - p[a]) * __builtin_amdgcn_rcpf(u[a]);
+                if (u[a] != F(0) && t < dmin) {
                     dmin = t;
-                    hit = b;
+                    hit = S.pbeg + (up ? S.fmax[a] : S.fmin[a]);
                 }
-                dist[i] = d;
-                rate[i] = nd;
+            }
+        } else {
+            // every lane evaluates every face: a reciprocal is cheaper than
+            // the divergence of skipping the faces it moves away from
+#pragma unroll
+            for (int i = 0; i < kMaxFaces; ++i) {
+                dist[i] = F(1);
+                rate[i] = F(0);
+                const int b = S.pbeg + i;
+                if (b < S.pend) {
+                    const DevPlane<F>& P = V.planes[b];
+                    const F nd = dev::dot3(P.n, L.dir);
+                    const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
+                                                                                __builtin_fmaf(P.n[0], L.pos.x, P.c)));
+                    const F t = -d * __builtin_amdgcn_rcpf(nd);
+                    if (nd < F(0) && t < dmin) {
+                        dmin = t;
+                        hit = b;
+                    }
+                    dist[i] = d;
+                    rate[i] = nd;
+                }
             }
         }
     } else {
@@ -714,7 +743,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         }
     }
     if (S.pore >= 0) {
-        const DevPore<F>& Q = K.pores[S.pore];
+        const DevPore<F>& Q = V.pores[S.pore];
         const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
         const F bq = dx * L.dir.x + dy * L.dir.y;
         if (bq < F(0)) {
@@ -742,13 +771,18 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     // kill test (subdomain.cpp:186-190): outside any face by more than eps
     bool outside;
     if constexpr (sizeof(F) == 4) {
-        // FP32: a face's distance after the move is d + dmin * rate
-        outside = false;
+        if (S.box) {
+            outside = L.pos.x < S.kmin[0] || L.pos.x > S.kmax[0] || L.pos.y < S.kmin[1] ||
+                      L.pos.y > S.kmax[1] || L.pos.z < S.kmin[2] || L.pos.z > S.kmax[2];
+        } else {
+            // a face's distance after the move is d + dmin * rate
+            outside = false;
 #pragma unroll
-        for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -S.eps;
-        if (S.pore >= 0) outside |= !outside_pore(K.pores[S.pore], L.pos);
+            for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -S.eps;
+        }
+        if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
     } else {
-        outside = !inside(V.planes, S, K.pores, L.pos);
+        outside = !inside(V.planes, S, V.pores, L.pos);
     }
     if (dmin < -S.eps || outside) {
         fate = MCG_ESCAPED;
@@ -782,7 +816,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                 scatter_fp32<F>(K, V, S, ev, b0, b1, hit, L, fate);
             } else {
                 L.nbdry++;
-                pore_scatter<F>(K.pores[S.pore], ev, b0, b1, L);
+                pore_scatter<F>(V.pores[S.pore], ev, b0, b1, L);
             }
         } else {
             if (hit == -1) {
@@ -820,7 +854,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                         } else {
                             for (int q = 0; q < P.icnt; ++q) {
                                 const int cand = K.inter_list[P.ibeg + q];
-                                if (inside(V.planes, V.sdoms[cand], K.pores, L.pos)) {
+                                if (inside(V.planes, V.sdoms[cand], V.pores, L.pos)) {
                                     target = cand;
                                     break;
                                 }
@@ -833,7 +867,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                 }
             } else {
                 L.nbdry++;
-                pore_scatter<F>(K.pores[S.pore], ev, b0, b1, L);
+                pore_scatter<F>(V.pores[S.pore], ev, b0, b1, L);
             }
         }
         if (fate == 0xFFFFFFFFu) {
@@ -938,28 +972,77 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     Lane<F> L;
     Walk<F> W;
     LaneStats st;
-    for (;;) {
-        const bool need = !active && !finished && !walking;
-        if (emit_now(need, active || walking)) {
-            const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
-            if (need) {
-                if (mine < K.count) {
-                    emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
+    if constexpr (sizeof(F) == 4) {
+        // FP32: each lane also holds its next bundle, emitted ahead of time
+        // (N, valid while `full`), so a lane whose bundle ends continues at
+        // once.  Emission runs in rounds in which every lane with an empty
+        // slot emits: a round waits for kSlotQuorum empty slots (or for
+        // kEmitQuorum lanes left with nothing to run, or for a wavefront with
+        // nothing in flight), so its Philox and sampling work runs on many
+        // lanes at a time instead of on the few that just ended.
+        Lane<F> N;
+        bool full = false, drained = false;
+        for (;;) {
+            if (!active && !walking && full) {
+                L = N;
+                full = false;
+                active = true;
+            }
+            const bool empty = !full && !drained;
+            const uint64_t me = __ballot(empty);
+            if (me != 0ull &&
+                (__popcll(me) >= kSlotQuorum || __popcll(__ballot(empty && !active && !walking)) >= kEmitQuorum ||
+                 __ballot(active || walking) == 0ull)) {
+                const int64_t mine = claim(empty, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
+                if (empty) {
+                    if (mine < K.count) {
+                        emit<F>(K, K.flux_alias, V.mfp, K.first + mine, N);
+                        full = true;
+                    } else {
+                        drained = true;
+                    }
+                }
+                if (!active && !walking && full) {
+                    L = N;
+                    full = false;
                     active = true;
-                } else {
-                    finished = true;
                 }
             }
-        }
-        if (__ballot(active || walking) == 0ull) break;
-        if (active && !walking) {
-            const uint32_t fate = scatter_step<F, kDefer>(K, V, sink, L, W, walking);
-            if (fate != 0xFFFFFFFFu) {
-                st.add(fate, L.nstep);
-                active = false;
+            if (__ballot(active || walking || full || !drained) == 0ull) break;
+            if (active && !walking) {
+                const uint32_t fate = scatter_step<F, kDefer>(K, V, sink, L, W, walking);
+                if (fate != 0xFFFFFFFFu) {
+                    st.add(fate, L.nstep);
+                    active = false;
+                }
             }
+            if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
         }
-        if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
+        (void)finished;
+    } else {
+        for (;;) {
+            const bool need = !active && !finished && !walking;
+            if (emit_now(need, active || walking)) {
+                const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
+                if (need) {
+                    if (mine < K.count) {
+                        emit<F>(K, K.flux_alias, V.mfp, K.first + mine, L);
+                        active = true;
+                    } else {
+                        finished = true;
+                    }
+                }
+            }
+            if (__ballot(active || walking) == 0ull) break;
+            if (active && !walking) {
+                const uint32_t fate = scatter_step<F, kDefer>(K, V, sink, L, W, walking);
+                if (fate != 0xFFFFFFFFu) {
+                    st.add(fate, L.nstep);
+                    active = false;
+                }
+            }
+            if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
+        }
     }
     st.publish(K.counters, lane);
     if (kLdsTally) {
@@ -1127,7 +1210,16 @@ void convert_sdom(const DevSdom<double>& s, DevSdom<F>& d, double eps_scale) {
     for (int i = 0; i < 3; ++i) { d.o[i] = F(s.o[i]); d.divf[i] = F(s.divf[i]); d.max[i] = s.max[i]; d.stride[i] = s.stride[i]; }
     for (int i = 0; i < 9; ++i) d.inv[i] = F(s.inv[i]);
     d.eps = F(s.eps * eps_scale);
-    d.accum = s.accum; d.pbeg = s.pbeg; d.pend = s.pend; d.col_off = s.col_off; d.pore = s.pore; d.pad = 0;
+    d.accum = s.accum; d.pbeg = s.pbeg; d.pend = s.pend; d.col_off = s.col_off; d.pore = s.pore;
+    d.box = s.box;
+    for (int i = 0; i < 3; ++i) {
+        d.bmin[i] = F(s.box ? s.bmin[i] : 0.0);
+        d.bmax[i] = F(s.box ? s.bmax[i] : 0.0);
+        d.kmin[i] = s.box ? d.bmin[i] - d.eps : F(0);
+        d.kmax[i] = s.box ? d.bmax[i] + d.eps : F(0);
+        d.fmin[i] = s.box ? s.fmin[i] : 0;
+        d.fmax[i] = s.box ? s.fmax[i] : 0;
+    }
 }
 
 template <typename F>
@@ -1208,6 +1300,7 @@ size_t lds_bytes(const HostScene& h, int rows, bool tables, bool tally) {
     size_t off = ((tables ? h.mode_cell.size() : 0) * (8 + 2 * sizeof(F)) + 15) & ~size_t(15);
     off = (off + sizeof(DevPlane<F>) * h.planes.size() + 15) & ~size_t(15);
     off = (off + sizeof(DevSdom<F>) * h.sdoms.size() + 15) & ~size_t(15);
+    off = (off + sizeof(DevPore<F>) * h.pores.size() + 15) & ~size_t(15);
     if (tally) off += sizeof(double) * (size_t(rows) * h.cols + (h.has_1d ? size_t(rows) * (h.cols + 1) : 0));
     return off;
 }

@@ -63,7 +63,10 @@ struct DevSdom {
     int32_t col_off;
     int32_t stride[3];
     int32_t pore;     // index of the cylindrical pore or -1
-    int32_t pad;
+    int32_t box;      // 1: six axis-aligned faces (the FP32 flight's box advect)
+    F bmin[3], bmax[3];        // box: the faces' positions on each axis
+    F kmin[3], kmax[3];        // box: positions beyond which a bundle is lost (eps outside)
+    int32_t fmin[3], fmax[3];  // box: plane offsets (from pbeg) of the low / high face
 };
 
 template <typename F>
