@@ -69,6 +69,11 @@ constexpr int kEmitQuorum = MCG_EMIT_QUORUM;
 #define MCG_SLOT_QUORUM 16
 #endif
 constexpr int kSlotQuorum = MCG_SLOT_QUORUM;
+// FP32 single solve: the end phase starts this many bundles per lane of the
+// grid before the last id
+#ifndef MCG_TAIL_LANES
+#define MCG_TAIL_LANES 2
+#endif
 
 #ifndef MCG_WALK_QUORUM
 #define MCG_WALK_QUORUM 12
@@ -106,6 +111,7 @@ struct KParams {
     int32_t lds_tally;
     int32_t lds_diff;      // the LDS tally holds the range-add difference array (1-D tallies)
     int32_t chunk;         // bundle ids a wavefront claims at a time
+    int64_t tail;          // FP32 single solve: ids from the end where claims shrink (0: never)
     int64_t trace_id;      // bundle whose steps are recorded, -1 for none
     double* trace;         // MCG_TRACE_WIDTH doubles per step
     int64_t trace_cap;     // records available
@@ -933,7 +939,8 @@ __device__ __forceinline__ void walk_phase(Walk<F>& W, bool& walking, bool has_w
 // offset for lanes with `need` (all lanes call, the pool stays wave-uniform).
 template <typename Counter>
 __device__ __forceinline__ int64_t claim(bool need, int lane, uint64_t lt_mask, int64_t& base,
-                                         int64_t& lim, Counter* counter, int chunk) {
+                                         int64_t& lim, Counter* counter, int chunk, int64_t count = 0,
+                                         int64_t tail = 0, int64_t* seen = nullptr) {
     const uint64_t mask = __ballot(need);
     if (!mask) return 0;
     const int cnt = __popcll(mask);
@@ -944,13 +951,25 @@ __device__ __forceinline__ int64_t claim(bool need, int lane, uint64_t lt_mask, 
         mine = base + rank;
         base += cnt;
     } else {
+        // within `tail` ids of the end a wavefront takes no more than its
+        // lanes need (at least one id per lane), so the last bundles spread
+        // over the wavefronts instead of waiting in a few private pools
         const int leader = __ffsll((long long)mask) - 1;
         unsigned long long fresh = 0;
-        if (lane == leader) fresh = (unsigned long long)atomicAdd(counter, Counter(chunk));
+        int want = chunk;
+        if (lane == leader) {
+            if (tail > 0) {
+                const unsigned long long cur = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (int64_t(cur) + tail >= count) want = max(cnt - int(avail), min(chunk, kWave));
+            }
+            fresh = (unsigned long long)atomicAdd(counter, Counter(want));
+        }
         fresh = __shfl(fresh, leader);
+        want = __shfl(want, leader);
         mine = rank < avail ? base + rank : int64_t(fresh) + (rank - avail);
         base = int64_t(fresh) + (cnt - avail);
-        lim = int64_t(fresh) + chunk;
+        lim = int64_t(fresh) + want;
+        if (seen) *seen = int64_t(fresh) + want;
     }
     return mine;
 }
@@ -982,18 +1001,23 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
         // lanes at a time instead of on the few that just ended.
         Lane<F> N;
         bool full = false, drained = false;
+        int64_t seen = 0;   // the claim counter when this wavefront last read it
         for (;;) {
             if (!active && !walking && full) {
                 L = N;
                 full = false;
                 active = true;
             }
-            const bool empty = !full && !drained;
+            // near the end only lanes with nothing to run fill a slot, so the
+            // last bundles go to idle lanes rather than queue behind live ones
+            const bool near_end = K.count - seen < K.tail;
+            const bool empty = !full && !drained && (!near_end || (!active && !walking));
             const uint64_t me = __ballot(empty);
             if (me != 0ull &&
                 (__popcll(me) >= kSlotQuorum || __popcll(__ballot(empty && !active && !walking)) >= kEmitQuorum ||
                  __ballot(active || walking) == 0ull)) {
-                const int64_t mine = claim(empty, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
+                const int64_t mine = claim(empty, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk, K.count,
+                                           K.tail, &seen);
                 if (empty) {
                     if (mine < K.count) {
                         emit<F>(K, K.flux_alias, V.mfp, K.first + mine, N);
@@ -1442,6 +1466,7 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     if (sc->tune.blocks_per_cu > 0) blocks_per_cu = std::min(blocks_per_cu, int(sc->tune.blocks_per_cu));
     const int64_t needed = (p.count + block - 1) / block;
     const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
+    K.tail = int64_t(MCG_TAIL_LANES) * grid * block;
     if (!sc->ev0) HIP_TRY(hipEventCreate(&sc->ev0));
     if (!sc->ev1) HIP_TRY(hipEventCreate(&sc->ev1));
     HIP_TRY(hipEventRecord(sc->ev0, 0));
