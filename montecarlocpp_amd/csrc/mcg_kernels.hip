@@ -987,7 +987,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     const int lane = tid & (kWave - 1);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
     int64_t pool_base = 0, pool_lim = 0;
-    bool active = false, finished = false, walking = false;
+    bool active = false, walking = false;
     Lane<F> L;
     Walk<F> W;
     LaneStats st;
@@ -1042,8 +1042,8 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
             }
             if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
         }
-        (void)finished;
     } else {
+        bool finished = false;
         for (;;) {
             const bool need = !active && !finished && !walking;
             if (emit_now(need, active || walking)) {
