@@ -1304,15 +1304,19 @@ struct mcg_scene {
     size_t lds_limit = 0;
     mcg_tune tune{0, 0, 0, 0, 0};   // launch shape for this scene's solves (0 = default)
     // per-solve device workspace, kept between solves (one thread solves a
-    // scene at a time): tallies (direct then difference array), counters,
-    // emitter CDF, timing events
+    // scene at a time): tallies (direct, difference array, then the
+    // counters, read back in one copy into pinned host memory), emitter CDF
+    // (uploaded again only when it changes), timing events
     mcg::DevBuf<double> ws_tally;
-    mcg::DevBuf<unsigned long long> ws_ctr;
     mcg::DevBuf<int64_t> ws_cdf;
+    std::vector<int64_t> ws_cdf_host;
+    double* pinned = nullptr;
+    size_t pinned_n = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~mcg_scene() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (pinned) (void)hipHostFree(pinned);
     }
 };
 
@@ -1415,13 +1419,24 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     const size_t nd = size_t(rows) * cols, nf = size_t(rows) * (cols + 1);
     DevBuf<uint32_t> cnt;
     int rc;
-    if ((rc = sc->ws_tally.reserve(nd + nf))) return rc;
-    if ((rc = sc->ws_ctr.reserve(C_COUNT))) return rc;
-    if ((rc = sc->ws_cdf.reserve(p.emit_cdf.size()))) return rc;
-    HIP_TRY(hipMemcpy(sc->ws_cdf.p, p.emit_cdf.data(), p.emit_cdf.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    const size_t nall = nd + nf + C_COUNT;   // counters: C_COUNT 8-byte words after the tallies
+    if ((rc = sc->ws_tally.reserve(nall))) return rc;
+    if (sc->ws_cdf.p == nullptr || sc->ws_cdf.n < p.emit_cdf.size() || sc->ws_cdf_host != p.emit_cdf) {
+        if ((rc = sc->ws_cdf.reserve(p.emit_cdf.size()))) return rc;
+        HIP_TRY(hipMemcpy(sc->ws_cdf.p, p.emit_cdf.data(), p.emit_cdf.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+        sc->ws_cdf_host = p.emit_cdf;
+    }
+    if (sc->pinned_n < nall) {
+        if (sc->pinned) HIP_TRY(hipHostFree(sc->pinned));
+        sc->pinned = nullptr;
+        sc->pinned_n = 0;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&sc->pinned), nall * sizeof(double), hipHostMallocDefault));
+        sc->pinned_n = nall;
+    }
     double* const direct = sc->ws_tally.p;
     double* const diff = direct + nd;
-    unsigned long long* const ctr = sc->ws_ctr.p;
+    unsigned long long* const ctr = reinterpret_cast<unsigned long long*>(direct + nd + nf);
     const bool want_counts = counts && counts->nscat && counts->nbdry && counts->nstep && counts->fate;
     if (want_counts && (rc = cnt.alloc(size_t(4) * p.count))) return rc;
     hipLaunchKernelGGL(zero_kernel, dim3(int(std::min<size_t>(1024, (nd + nf + 255) / 256 + 1))), dim3(256), 0, 0,
@@ -1474,15 +1489,15 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, 0));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(sc->ev1, 0));
-    HIP_TRY(hipEventSynchronize(sc->ev1));
+    HIP_TRY(hipMemcpyAsync(sc->pinned, direct, nall * sizeof(double), hipMemcpyDeviceToHost, 0));
+    HIP_TRY(hipStreamSynchronize(0));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, sc->ev0, sc->ev1));
 
-    std::vector<double> hd(nd), hf(nf);
+    const double* hd = sc->pinned;
+    const double* hf = sc->pinned + nd;
     unsigned long long hc[C_COUNT];
-    HIP_TRY(hipMemcpy(hd.data(), direct, nd * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hf.data(), diff, nf * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
+    std::memcpy(hc, sc->pinned + nd + nf, sizeof(hc));
     if (want_counts) {
         HIP_TRY(hipMemcpy(counts->nscat, cnt.p, p.count * 4, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(counts->nbdry, cnt.p + p.count, p.count * 4, hipMemcpyDeviceToHost));
@@ -1491,7 +1506,7 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     }
     if (trace && trace_cap > 0)
         HIP_TRY(hipMemcpy(trace, tbuf.p, tbuf.n * sizeof(double), hipMemcpyDeviceToHost));
-    finalise_field(h, p, hd.data(), hf.data(), out);
+    finalise_field(h, p, hd, hf, out);
     if (stats) {
         stats->bundles = int64_t(hc[C_BUNDLES]);
         stats->steps = int64_t(hc[C_STEPS]);
