@@ -212,3 +212,22 @@ def test_fp32_same_streams_close(si300):
     assert np.mean(c32.nscat == c64.nscat) > 0.99
     assert abs(f32[1].mean() / f64[1].mean() - 1) < 1e-4
     assert np.linalg.norm(f32[0] - f64[0]) / np.linalg.norm(f64[0]) < 1e-3
+
+
+def test_fp32_large_solve_every_bundle_once(si300):
+    """A solve larger than the end phase (two bundles per lane of the grid):
+    next-bundle slots fill until the last ids, then claims shrink.  Every
+    bundle id runs exactly once (each has a recorded fate and at least one
+    step, and the steps add up to the solve's count), and the heat flux is
+    that of FP64 on the same streams to 1e-3."""
+    dom = scenes.bar(1e-6, 1e-7, 1e-7, 100, 10.0, -10.0)
+    n = 1_500_000
+    with engine.Scene(si300, dom) as sc:
+        f32, s32, c32 = sc.solve(scenes.problem(n, seed=7, precision=abi.FP32_FLIGHT), counts=True)
+        f64, s64, _ = sc.solve(scenes.problem(n, seed=7, precision=abi.FP64))
+    assert s32["bundles"] == n
+    assert np.all(c32.nstep >= 1)
+    assert int(c32.nstep.astype(np.int64).sum()) == s32["steps"]
+    assert np.all(c32.fate <= 3)
+    assert s32["escapes"] <= n // 1000
+    assert abs(f32[1].mean() / f64[1].mean() - 1) < 1e-3
