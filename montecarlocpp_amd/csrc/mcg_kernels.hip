@@ -737,14 +737,35 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     } else {
         (void)dist;
         (void)rate;
-        for (int b = S.pbeg; b < S.pend; ++b) {
-            const DevPlane<F>& P = V.planes[b];
-            const F nd = dev::dot3(P.n, L.dir);
-            if (nd >= F(0)) continue;
-            const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
-            if (t < dmin) {
-                dmin = t;
-                hit = b;
+        if (S.box) {
+            // the checker's face loop evaluated for the face ahead on each
+            // axis only: the axis's other face has n.u >= 0 (exactly: the
+            // normal's other components are zeros) and the loop skips it;
+            // equal parameters keep the lower face index, as the loop in face
+            // order does, and a face only displaces the intrinsic event when
+            // strictly nearer
+            const F u[3] = {L.dir.x, L.dir.y, L.dir.z};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (u[a] == F(0)) continue;
+                const int b = S.pbeg + (u[a] > F(0) ? S.fmax[a] : S.fmin[a]);
+                const DevPlane<F>& P = V.planes[b];
+                const F t = -(P.c + dev::dot3(P.n, L.pos)) / dev::dot3(P.n, L.dir);
+                if (t < dmin || (t == dmin && hit >= 0 && b < hit)) {
+                    dmin = t;
+                    hit = b;
+                }
+            }
+        } else {
+            for (int b = S.pbeg; b < S.pend; ++b) {
+                const DevPlane<F>& P = V.planes[b];
+                const F nd = dev::dot3(P.n, L.dir);
+                if (nd >= F(0)) continue;
+                const F t = -(P.c + dev::dot3(P.n, L.pos)) / nd;
+                if (t < dmin) {
+                    dmin = t;
+                    hit = b;
+                }
             }
         }
     }
@@ -786,6 +807,13 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
 #pragma unroll
             for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -S.eps;
         }
+        if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
+    } else if (S.box) {
+        // the checker's test face by face: n.p + c is p_a - bmin_a or
+        // bmax_a - p_a exactly (the normal's other components are zeros)
+        outside = L.pos.x - S.bmin[0] < -S.eps || S.bmax[0] - L.pos.x < -S.eps ||
+                  L.pos.y - S.bmin[1] < -S.eps || S.bmax[1] - L.pos.y < -S.eps ||
+                  L.pos.z - S.bmin[2] < -S.eps || S.bmax[2] - L.pos.z < -S.eps;
         if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
     } else {
         outside = !inside(V.planes, S, V.pores, L.pos);
