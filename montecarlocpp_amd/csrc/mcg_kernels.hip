@@ -853,23 +853,36 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                 pore_scatter<F>(V.pores[S.pore], ev, b0, b1, L);
             }
         } else {
-            if (hit == -1) {
-                L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b0.w);
-                L.dir = dev::normalized(dev::draw_iso<F>(b0, b1));
+            const bool intr = hit == -1;
+            const DevPlane<F>& P = V.planes[hit >= 0 ? hit : S.pbeg];
+            int kind = P.kind;
+            if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+            if (intr || (hit >= 0 && kind == MCG_DIFF)) {
+                // drawIso (intrinsic) and drawAniso (diffuse face) with each
+                // lane's own arithmetic, exactly: the same uniforms, one
+                // square root for sin(theta), one sin/cos(pi q), so a
+                // wavefront whose lanes take both evaluates them once
+                const F r = dev::Real<F>::uniform(b0.x, b1.x) * F(2) + F(-1);
+                const F ar = r < F(0) ? -r : r;
+                const F sinT = dev::Real<F>::sqrt_(intr ? F(1) - r * r : ar);
+                const F cosT = intr ? r : dev::Real<F>::sqrt_(F(1) - ar);
+                const F q = dev::Real<F>::uniform(b0.y, b1.y) * F(2) + F(-1);
+                F sq, cq;
+                dev::Real<F>::sincos_pi(q, sq, cq);
+                const dev::V3<F> v{sinT * cq, sinT * sq, cosT};
+                L.dir = dev::normalized(intr ? v : dev::matvec(P.rot, v));   // boundary.cpp:308-312
                 L.nscat++;
+            }
+            if (intr) {
+                L.mode = dev::alias_draw(V.scat, uint32_t(K.nmodes), b0.w);
                 L.scat_next = draw_path<F>(ev, b0, b1, V.mfp[L.mode]);
             } else if (hit >= 0) {
                 L.nbdry++;
-                const DevPlane<F>& P = V.planes[hit];
-                int kind = P.kind;
-                if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
                 switch (kind) {
                     case MCG_SPEC:   // boundary.cpp:283-287
                         L.dir = dev::normalized(dev::matvec(P.refl, L.dir));
                         break;
-                    case MCG_DIFF:   // boundary.cpp:308-312
-                        L.dir = dev::normalized(dev::matvec(P.rot, dev::draw_aniso<F>(b0, b1, false)));
-                        L.nscat++;
+                    case MCG_DIFF:   // direction drawn above
                         break;
                     case MCG_ISOT:   // boundary.cpp:456-460
                         fate = MCG_ABSORBED;
