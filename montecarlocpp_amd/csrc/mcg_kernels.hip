@@ -375,16 +375,13 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         const F ecd_f = S.divf[flag] * dev::matrow(S.inv, flag, lb);
         const int bb = min(max(int(floor(bcd_f)), 0), S.max[flag]);
         const int ee = min(max(int(floor(ecd_f)), 0), S.max[flag]);
-        if (sizeof(F) == 8 && bb == ee) {
-#pragma unroll
-            for (int j = 0; j < kMaxRows; ++j)
-                if (j < nv) sink.add(r0 + j, col0 + bb, amt[j]);
-            return false;
-        }
         const int lo = bb < ee ? bb + 1 : ee + 1;
         const int hi = bb < ee ? ee - 1 : bb - 1;
         if constexpr (sizeof(F) == 8) {
-            // the checker's FP64 arithmetic: each amount divided by |dc|
+            // the checker's FP64 arithmetic: a segment inside one cell adds
+            // its amounts there, a longer one each amount divided by |dc|;
+            // both kinds of lane share the adds to the first cell
+            const bool one = bb == ee;
             const double bcd = double(bcd_f), ecd = double(ecd_f);
             const double ad = fabs(double(ecd_f - bcd_f));
             const double fb = bb < ee ? double(1 + bb) - bcd : bcd - double(bb);
@@ -393,8 +390,8 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             for (int j = 0; j < kMaxRows; ++j) {
                 if (j >= nv) continue;
                 const double ca = amt[j] / ad;
-                sink.add(r0 + j, col0 + bb, ca * fb);
-                sink.add(r0 + j, col0 + ee, ca * fe);
+                sink.add(r0 + j, col0 + bb, one ? amt[j] : ca * fb);
+                if (!one) sink.add(r0 + j, col0 + ee, ca * fe);
                 if (lo <= hi) sink.add_range(r0 + j, col0 + lo, col0 + hi, ca);
             }
         } else {
