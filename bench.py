@@ -145,6 +145,7 @@ def main():
         "temperature_stat_rel_se": float(np.linalg.norm(Tprof.std(axis=0, ddof=1))
                                          / np.sqrt(len(q)) / np.linalg.norm(Tprof.mean(axis=0))),
         "k_eff_W_per_mK": float(q.mean() * 1e-6 / 20.0),
+        "k_eff_fp32_vs_fp64": abs(f32[1].mean() - f64[1].mean()) / abs(f64[1].mean()),
     }
 
     cpu = None
@@ -224,7 +225,12 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
     mat, dom, prob = workloads.porous_film(nbundles)
     vol = engine.cell_volumes(dom)
     with engine.Scene(mat, dom) as sc:
-        sc.solve(workloads.porous_film(nbundles // 10)[2])
+        # warm-up, kept for the precision split: the same 1e7-bundle streams
+        # in FP64 (the deterministic mode, equal to the FP64 oracle)
+        f_warm32, _, _ = sc.solve(workloads.porous_film(nbundles // 10)[2])
+        f_warm64, _, _ = sc.solve(workloads.porous_film(nbundles // 10, precision=abi.FP64)[2])
+        # the statistical noise of a 1e7-bundle solve, for scale: FP32, another seed
+        f_seed2, _, _ = sc.solve(workloads.porous_film(nbundles // 10, seed=2)[2])
         runs = []
         for k in range(2):
             p = workloads.porous_film(nbundles, seed=1 + k)[2]
@@ -243,6 +249,18 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
            "steps_per_s": steps / wall, "kernel_steps_per_s": steps / kern,
            "ms_per_solve": wall / len(runs) * 1e3,
            "k_eff_W_per_mK": float(np.mean(keff)), "k_eff_seed_spread": float(abs(keff[0] - keff[1])),
+           # FP32 flight against FP64 (same streams, 1e7 bundles); the diffuse
+           # walls decorrelate the two after a few bounces, so the difference
+           # is set by the solve's noise, shown beside it (seed 1 vs seed 2)
+           "precision_error_1e7_bundles": {
+               "k_eff_fp32_vs_fp64": abs(workloads.k_eff_porous(f_warm32, vol) / workloads.k_eff_porous(f_warm64, vol)
+                                         - 1.0),
+               "flux_x_map_fp32_vs_fp64": rel(f_warm32[1], f_warm64[1]),
+               "temperature_map_fp32_vs_fp64": rel(f_warm32[0], f_warm64[0]),
+               "k_eff_seed1_vs_seed2": abs(workloads.k_eff_porous(f_warm32, vol) / workloads.k_eff_porous(f_seed2, vol)
+                                           - 1.0),
+               "flux_x_map_seed1_vs_seed2": rel(f_warm32[1], f_seed2[1]),
+               "temperature_map_seed1_vs_seed2": rel(f_warm32[0], f_seed2[0])},
            "roofline_frac_96B": steps / kern * BYTES_PER_STEP_FP32 / HBM_PEAK}
     if cpu:
         from oracle import oracle  # CPU baseline leg only
