@@ -76,7 +76,7 @@ constexpr int kSlotQuorum = MCG_SLOT_QUORUM;
 #endif
 
 #ifndef MCG_WALK_QUORUM
-#define MCG_WALK_QUORUM 12
+#define MCG_WALK_QUORUM 20
 #endif
 constexpr int kWalkQuorum = MCG_WALK_QUORUM;
 
