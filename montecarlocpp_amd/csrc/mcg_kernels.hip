@@ -63,6 +63,12 @@ constexpr uint32_t kMaxPosRedraw = 4096;
 #define MCG_EMIT_QUORUM 8
 #endif
 constexpr int kEmitQuorum = MCG_EMIT_QUORUM;
+// FP64 single solve: its emission is longer (four Philox blocks and the
+// portable transcendentals), so its lanes wait for more peers
+#ifndef MCG_EMIT_QUORUM_FP64
+#define MCG_EMIT_QUORUM_FP64 16
+#endif
+constexpr int kEmitQuorumFp64 = MCG_EMIT_QUORUM_FP64;
 // FP32 single solve: empty next-bundle slots a wavefront collects before it
 // emits into them
 #ifndef MCG_SLOT_QUORUM
@@ -949,10 +955,11 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
 }
 
 // Whether the wavefront claims and emits this iteration (wave-uniform).
+template <int kQuorum = kEmitQuorum>
 __device__ __forceinline__ bool emit_now(bool need, bool busy) {
     const uint64_t m = __ballot(need);
     if (m == 0ull) return false;
-    return __popcll(m) >= kEmitQuorum || __ballot(busy) == 0ull;
+    return __popcll(m) >= kQuorum || __ballot(busy) == 0ull;
 }
 
 // The wavefront's walk phase: every walking lane advances one crossing per
@@ -1084,7 +1091,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
         bool finished = false;
         for (;;) {
             const bool need = !active && !finished && !walking;
-            if (emit_now(need, active || walking)) {
+            if (emit_now<kEmitQuorumFp64>(need, active || walking)) {
                 const int64_t mine = claim(need, lane, lt_mask, pool_base, pool_lim, K.next, K.chunk);
                 if (need) {
                     if (mine < K.count) {
