@@ -13,14 +13,17 @@ Printed: one JSON line with the scatter-step throughput, the roofline
 object of the trajectory kernel (algorithmic bytes = 96 B of bundle state
 round trip per scatter-step), the tally error of the FP32-flight run against
 the FP64 run of the same streams, the FP64 CPU checker timed on a bounded
-sample of the same workload on the host cores, and two secondary objects:
-"batched" (256 independent slab solves in one persistent launch against the
-same solves run one mcg_solve at a time) and "serving" (an open-loop Poisson
-request stream through the continuous-batching server, with p50/p99
-latency), plus "porous_film" (the periodic porous film with a cylindrical
-pore, 1e8 bundles, 64 x 64 map, with its own CPU baseline sample).  --headline-only skips the secondary objects and the CPU leg.
+sample of the same workload on the host cores, and secondary objects:
+"slab" (the 1-D slab of 1e5 bundles that the reference runs on its CPU, in
+deterministic FP64, beside the whole solve on the CPU checker), "porous_film"
+(the periodic porous film with a cylindrical pore, 1e8 bundles, 64 x 64 map,
+with its own CPU baseline sample and FP32-vs-FP64 errors), "batched" (256
+independent slab solves in one persistent launch against the same solves run
+one mcg_solve at a time) and "serving" (an open-loop Poisson request stream
+through the continuous-batching server, with p50/p99 latency).
+--headline-only skips the secondary objects and the CPU leg.
 
-    python bench.py --only porous_film|batched|serving
+    python bench.py --only slab|porous_film|batched|serving
 
 runs one secondary object alone (for profiling it).
 
@@ -80,11 +83,12 @@ def main():
                     default=os.path.join(ROOT, "profiles", "trajectory_kernel_bar1e7_fp32_traffic.json"),
                     help="counter-measured HBM bytes per launch of the kernel (rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE, scaled by the copy-probe calibration)")
-    ap.add_argument("--only", choices=("porous_film", "batched", "serving"),
+    ap.add_argument("--only", choices=("slab", "porous_film", "batched", "serving"),
                     help="run one secondary workload alone and print its JSON object")
     args = ap.parse_args()
     if args.only:
-        run = {"porous_film": lambda: bench_porous(cpu=not args.no_cpu),
+        run = {"slab": lambda: bench_slab(cpu=not args.no_cpu),
+               "porous_film": lambda: bench_porous(cpu=not args.no_cpu),
                "batched": bench_batched, "serving": bench_serving}[args.only]
         print(json.dumps({args.only: run()}))
         return
@@ -168,8 +172,10 @@ def main():
         with open(args.traffic_json) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    batched = serving = porous = None
+    batched = serving = porous = slab = None
     if not args.headline_only:
+        slab = bench_slab(cpu=not args.no_cpu)
+        progress("slab done")
         porous = bench_porous(cpu=not args.no_cpu)
         progress("porous film done")
         batched = bench_batched()
@@ -211,6 +217,7 @@ def main():
         },
         "tally_error": tally_error,
         "cpu_baseline": cpu,
+        "slab": slab,
         "porous_film": porous,
         "batched": batched,
         "serving": serving,
@@ -272,6 +279,37 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
         out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s",
                                "cores": threads, "kind": "port",
                                "sample": f"{cpu_sample} bundles of the same porous film, FP64"}
+    return out
+
+
+def bench_slab(cpu=True, reps=20):
+    """The reference's own CPU-runnable case: the 1-D cross-plane slab
+    (100 nm, walls at +/-1 K, 50 bins, 1e5 bundles, seed 1), deterministic FP64
+    on the GPU (bundle for bundle the oracle's result) and, for comparison,
+    the whole solve on the CPU oracle."""
+    from montecarlocpp_amd import workloads
+    mat, dom, prob = workloads.slab()
+    with engine.Scene(mat, dom) as sc:
+        sc.solve(prob)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f, st, _ = sc.solve(prob)
+        wall = (time.perf_counter() - t0) / reps
+    out = {"workload": "1-D slab 100 nm, isothermal walls +/-1 K, specular sides, Si-like at 300 K, "
+                       "1e5 bundles, 50 bins, FP64 deterministic mode",
+           "bundles": st["bundles"], "scatter_steps": st["steps"],
+           "ms_per_solve": wall * 1e3, "kernel_ms": st["kernel_ms"],
+           "steps_per_s": st["steps"] / wall}
+    if cpu:
+        from oracle import oracle  # CPU baseline leg only
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        t0 = time.perf_counter()
+        fo, cst, _ = oracle.solve(mat, dom, prob, threads=threads)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s", "cores": threads,
+                               "kind": "port", "sample": "the whole 1e5-bundle solve, FP64",
+                               "ms_per_solve": ct * 1e3}
+        out["gpu_vs_cpu_tally_rel_diff"] = rel(f, fo)
     return out
 
 
