@@ -88,8 +88,15 @@ enum mcg_bdry_kind {
 enum mcg_cell_kind {
     MCG_PARALLELEPIPED = 0,   /* subdomain.h:128-198, faces Bac Lef Bot Fro Rig Top */
     MCG_TRIPRISM = 1,         /* subdomain.h:206-277, faces Bac Lef Bot Dia Top     */
-    MCG_TETRAHEDRON = 2       /* subdomain.h:285-352, faces Bac Lef Bot Dia         */
+    MCG_TETRAHEDRON = 2,      /* subdomain.h:285-352, faces Bac Lef Bot Dia         */
+    MCG_PRISM = 3,            /* subdomain.h:364-492, faces Bot Top Sid0..Sid(N-1)  */
+    MCG_PYRAMID = 4           /* subdomain.h:504-630, faces Bot Sid0..Sid(N-1)      */
 };
+
+/* Sides of the N-gon base of a prism or pyramid: 4 <= N <= MCG_MAX_SIDES
+ * (Polygon<N> is instantiated for N = 4..9, boundary.cpp:253-258). */
+#define MCG_MAX_SIDES 9
+#define MCG_MAX_FACES (MCG_MAX_SIDES + 2)
 
 /* One face of a cell: the boundary type chosen by the template argument in
  * the reference (e.g. Parallelepiped<IsotBoundary<Parallelogram>, Spec,
@@ -102,15 +109,20 @@ typedef struct mcg_face_spec {
 } mcg_face_spec;
 
 /* A convex cell: the constructor arguments (o, mat, div, gradT, T) of the
- * reference's cell templates. */
+ * reference's cell templates.  The parallelepiped, triangular prism and
+ * tetrahedron take three columns of mat and div per axis.  A prism or
+ * pyramid with an N-gon base takes N columns: column 0 is the prism's axis
+ * (the pyramid's apex), columns 1..N-1 the base's vertices after the origin,
+ * all relative to the origin; as in the reference (subdomain.h:388-391) its
+ * tally is one cell, or none when div[0] < 0 (div[1], div[2] are unused). */
 typedef struct mcg_subdomain_spec {
     int32_t cell;              /* mcg_cell_kind                                 */
-    int32_t nface;             /* 6, 5 or 4 for the three cell kinds            */
+    int32_t nface;             /* 6, 5, 4; N + 2 (prism) or N + 1 (pyramid)     */
     double origin[3];
-    double mat[9];             /* column-major: columns are the cell edges      */
+    double mat[3 * MCG_MAX_SIDES]; /* column-major: columns are the cell edges  */
     int64_t div[3];            /* tally divisions (subdomain.cpp:41-71)         */
     double grad_t[3];          /* imposed gradient: volumetric emitter          */
-    mcg_face_spec face[6];     /* in the reference's container order            */
+    mcg_face_spec face[MCG_MAX_FACES]; /* in the reference's container order    */
 } mcg_subdomain_spec;
 
 /* makePair for periodic faces (boundary.cpp:524-550: translation and

@@ -11,8 +11,10 @@ MCG_OK, MCG_EINVAL, MCG_ENOMEM, MCG_EDEVICE, MCG_ERANGE, MCG_EAGAIN, MCG_ENOTFOU
 
 # boundary kinds (boundary.h:137-281 plus the partial-specularity extension)
 SPEC, DIFF, ISOT, PERI, INTER, PARTIAL = range(6)
-# cell kinds (subdomain.h:128-352)
-PARALLELEPIPED, TRIPRISM, TETRAHEDRON = range(3)
+# cell kinds (subdomain.h:128-630); PRISM / PYRAMID have an N-gon base
+PARALLELEPIPED, TRIPRISM, TETRAHEDRON, PRISM, PYRAMID = range(5)
+MAX_SIDES = 9                  # N of a prism's or pyramid's base: 4..MAX_SIDES
+MAX_FACES = MAX_SIDES + 2
 # tally kinds (problem.h:157-239)
 TEMP, FLUX, MULTI, CUMTEMP, CUMFLUX = range(5)
 # precision
@@ -21,6 +23,15 @@ FP64, FP32_FLIGHT = range(2)
 ABSORBED, ESCAPED, MAXSCAT, MAXLOOP = range(4)
 
 NFACE = {PARALLELEPIPED: 6, TRIPRISM: 5, TETRAHEDRON: 4}
+
+
+def nface(cell, nside=None):
+    """Faces of a cell kind; a prism or pyramid needs its base's sides."""
+    if cell == PRISM:
+        return nside + 2
+    if cell == PYRAMID:
+        return nside + 1
+    return NFACE[cell]
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
@@ -44,8 +55,8 @@ class FaceSpec(C.Structure):
 
 class SubdomainSpec(C.Structure):
     _fields_ = [("cell", C.c_int32), ("nface", C.c_int32), ("origin", C.c_double * 3),
-                ("mat", C.c_double * 9), ("div", C.c_int64 * 3), ("grad_t", C.c_double * 3),
-                ("face", FaceSpec * 6)]
+                ("mat", C.c_double * (3 * MAX_SIDES)), ("div", C.c_int64 * 3),
+                ("grad_t", C.c_double * 3), ("face", FaceSpec * MAX_FACES)]
 
 
 class PairSpec(C.Structure):

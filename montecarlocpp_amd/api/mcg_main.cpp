@@ -39,7 +39,7 @@ void dump_scene(const mcg_domain_spec& d) {
         const mcg_subdomain_spec& c = d.sdom[s];
         std::cout << (s ? ", " : "") << "{\"cell\": " << c.cell << ", \"origin\": [" << c.origin[0]
                   << ", " << c.origin[1] << ", " << c.origin[2] << "], \"mat\": [";
-        for (int i = 0; i < 9; ++i) std::cout << (i ? ", " : "") << c.mat[i];
+        for (int i = 0; i < 3 * MCG_MAX_SIDES; ++i) std::cout << (i ? ", " : "") << c.mat[i];
         std::cout << "], \"div\": [" << c.div[0] << ", " << c.div[1] << ", " << c.div[2]
                   << "], \"grad_t\": [" << c.grad_t[0] << ", " << c.grad_t[1] << ", " << c.grad_t[2]
                   << "], \"faces\": [";

@@ -26,6 +26,8 @@
 #include <map>
 #include <memory>
 #include <random>
+#include <tuple>
+#include <utility>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -86,6 +88,22 @@ struct Matrix3d {
 };
 
 using VectorXd = std::vector<double>;
+
+// 3 x N, column-major like Eigen's Matrix3Xd
+class Matrix3Xd {
+    std::vector<double> d_;
+
+public:
+    Matrix3Xd() = default;
+    Matrix3Xd(long rows, long cols) : d_(size_t(3 * cols), 0.0) {
+        if (rows != 3) throw mcg::Error(MCG_EINVAL, "Matrix3Xd has three rows");
+    }
+    long cols() const { return long(d_.size() / 3); }
+    double& operator()(int r, int c) { return d_[size_t(3 * c + r)]; }
+    double operator()(int r, int c) const { return d_[size_t(3 * c + r)]; }
+    Vector3d col(int c) const { return {d_[3 * c], d_[3 * c + 1], d_[3 * c + 2]}; }
+    const double* data() const { return d_.data(); }
+};
 
 // Column-major rows x cols array (Eigen's ArrayXXd layout).
 class ArrayXXd {
@@ -234,6 +252,19 @@ public:
     Triangle() = default;
     Triangle(const Vector3d& i, const Vector3d& j) : i_(i), j_(j) {}
 };
+// Polygon<N> (boundary.h:112-130): the N - 1 vertices after the first,
+// relative to it; the engine checks the fan when the scene is built.
+template <int N>
+class Polygon : public Boundary::Shape {
+    static_assert(N > 3, "Polygon must have more than 3 sides");
+    Matrix3Xd verts_;
+
+public:
+    Polygon() = default;
+    explicit Polygon(const Matrix3Xd& v) : verts_(v) {
+        if (v.cols() != N - 1) throw mcg::Error(MCG_EINVAL, "Incorrect number of vertices");
+    }
+};
 
 // Non-emitting faces: a temperature is an error (boundary.cpp:272-326).
 template <int K>
@@ -335,7 +366,10 @@ public:
             s.div[k] = div_(k);
             s.grad_t[k] = gradT_(k);
         }
-        for (int i = 0; i < 9; ++i) s.mat[i] = mat_.m[i];
+        if (matx_.cols() > 0)
+            for (long i = 0; i < 3 * matx_.cols(); ++i) s.mat[i] = matx_.data()[i];
+        else
+            for (int i = 0; i < 9; ++i) s.mat[i] = mat_.m[i];
         std::vector<const Boundary*> b = bdryPtrs();
         s.nface = int32_t(b.size());
         for (size_t f = 0; f < b.size(); ++f) {
@@ -350,6 +384,7 @@ protected:
     int cell_ = MCG_PARALLELEPIPED;
     Vector3d o_, gradT_;
     Matrix3d mat_;
+    Matrix3Xd matx_;   // Prism / Pyramid: the N columns of their mat
     Vector3l div_;
 };
 using EmitSubdomain = Subdomain;
@@ -448,6 +483,127 @@ public:
     }
     std::vector<const Boundary*> bdryPtrs() const override {
         return {&std::get<0>(b_), &std::get<1>(b_), &std::get<2>(b_), &std::get<3>(b_)};
+    }
+};
+
+// Prism<Bot, Top, std::tuple<Sid0, ..., Sid(N-1)>>(o, mat, div, gradT, T)
+// (subdomain.h:364-492; the reference takes a fusion::vector of sides): an
+// N-gon base at o with vertices o + mat.col(1..N-1), the top at
+// o + mat.col(0), side k a parallelogram from vertex k to vertex k + 1.
+template <typename Bot, typename Top, typename Sid>
+class Prism;
+template <typename Bot, typename Top, typename... Sid>
+class Prism<Bot, Top, std::tuple<Sid...>> : public Subdomain {
+    static constexpr int N = int(sizeof...(Sid));
+    static_assert(N >= 4 && N <= MCG_MAX_SIDES, "a prism base has 4 to 9 sides");
+    using Cont = std::tuple<Bot, Top, Sid...>;
+    Cont b_;
+
+    static Matrix3Xd verts(const Matrix3Xd& mat, bool reversed) {
+        Matrix3Xd v(3, N - 1);
+        for (int k = 0; k < N - 1; ++k)
+            for (int r = 0; r < 3; ++r) v(r, k) = mat(r, reversed ? N - 1 - k : k + 1);
+        return v;
+    }
+    template <size_t... I>
+    static Cont make(const Vector3d& o, const Matrix3Xd& mat, const std::vector<double>& T,
+                     std::index_sequence<I...>) {
+        return Cont(Bot(o, Polygon<N>(verts(mat, false)), T.at(0)),
+                    Top(o + mat.col(0), Polygon<N>(verts(mat, true)), T.at(1)),
+                    Sid(o + (I == 0 ? Vector3d::Zero() : mat.col(int(I))),
+                        Parallelogram(mat.col(0), (I != N - 1 ? mat.col(int(I) + 1) : Vector3d::Zero()) -
+                                                      (I == 0 ? Vector3d::Zero() : mat.col(int(I)))),
+                        T.at(2 + I))...);
+    }
+
+public:
+    Prism(const Vector3d& o, const Matrix3Xd& mat, long div, const Vector3d& gradT = Vector3d::Zero(),
+          const std::vector<double>& T = std::vector<double>(N + 2, 0.0))
+        : b_(make(o, check(mat, T), T, std::index_sequence_for<Sid...>{})) {
+        cell_ = MCG_PRISM;
+        o_ = o;
+        matx_ = mat;
+        div_ = Vector3l(div, 0, 0);
+        gradT_ = gradT;
+    }
+    Prism(const Prism&) = delete;
+    template <int I>
+    typename std::tuple_element<I, Cont>::type& bdry() {
+        return std::get<I>(b_);
+    }
+    std::vector<const Boundary*> bdryPtrs() const override {
+        return ptrs(std::make_index_sequence<N + 2>{});
+    }
+
+private:
+    static const Matrix3Xd& check(const Matrix3Xd& mat, const std::vector<double>& T) {
+        if (mat.cols() != N) throw mcg::Error(MCG_EINVAL, "Incorrect number of matrix columns");
+        if (T.size() != size_t(N + 2)) throw mcg::Error(MCG_EINVAL, "Incorrect number of temperatures");
+        return mat;
+    }
+    template <size_t... I>
+    std::vector<const Boundary*> ptrs(std::index_sequence<I...>) const {
+        return {static_cast<const Boundary*>(&std::get<I>(b_))...};
+    }
+};
+
+// Pyramid<Bot, std::tuple<Sid0, ..., Sid(N-1)>>(o, mat, div, gradT, T)
+// (subdomain.h:504-630): an N-gon base at o, apex o + mat.col(0), side k a
+// triangle from vertex k to vertex k + 1 and the apex.
+template <typename Bot, typename Sid>
+class Pyramid;
+template <typename Bot, typename... Sid>
+class Pyramid<Bot, std::tuple<Sid...>> : public Subdomain {
+    static constexpr int N = int(sizeof...(Sid));
+    static_assert(N >= 4 && N <= MCG_MAX_SIDES, "a pyramid base has 4 to 9 sides");
+    using Cont = std::tuple<Bot, Sid...>;
+    Cont b_;
+
+    static Matrix3Xd verts(const Matrix3Xd& mat) {
+        Matrix3Xd v(3, N - 1);
+        for (int k = 0; k < N - 1; ++k)
+            for (int r = 0; r < 3; ++r) v(r, k) = mat(r, k + 1);
+        return v;
+    }
+    template <size_t... I>
+    static Cont make(const Vector3d& o, const Matrix3Xd& mat, const std::vector<double>& T,
+                     std::index_sequence<I...>) {
+        return Cont(Bot(o, Polygon<N>(verts(mat)), T.at(0)),
+                    Sid(o + (I == 0 ? Vector3d::Zero() : mat.col(int(I))),
+                        Triangle(mat.col(0) - (I == 0 ? Vector3d::Zero() : mat.col(int(I))),
+                                 (I != N - 1 ? mat.col(int(I) + 1) : Vector3d::Zero()) -
+                                     (I == 0 ? Vector3d::Zero() : mat.col(int(I)))),
+                        T.at(1 + I))...);
+    }
+
+public:
+    Pyramid(const Vector3d& o, const Matrix3Xd& mat, long div, const Vector3d& gradT = Vector3d::Zero(),
+            const std::vector<double>& T = std::vector<double>(N + 1, 0.0))
+        : b_(make(o, check(mat, T), T, std::index_sequence_for<Sid...>{})) {
+        cell_ = MCG_PYRAMID;
+        o_ = o;
+        matx_ = mat;
+        div_ = Vector3l(div, 0, 0);
+        gradT_ = gradT;
+    }
+    Pyramid(const Pyramid&) = delete;
+    template <int I>
+    typename std::tuple_element<I, Cont>::type& bdry() {
+        return std::get<I>(b_);
+    }
+    std::vector<const Boundary*> bdryPtrs() const override {
+        return ptrs(std::make_index_sequence<N + 1>{});
+    }
+
+private:
+    static const Matrix3Xd& check(const Matrix3Xd& mat, const std::vector<double>& T) {
+        if (mat.cols() != N) throw mcg::Error(MCG_EINVAL, "Incorrect number of matrix columns");
+        if (T.size() != size_t(N + 1)) throw mcg::Error(MCG_EINVAL, "Incorrect number of temperatures");
+        return mat;
+    }
+    template <size_t... I>
+    std::vector<const Boundary*> ptrs(std::index_sequence<I...>) const {
+        return {static_cast<const Boundary*>(&std::get<I>(b_))...};
     }
 };
 

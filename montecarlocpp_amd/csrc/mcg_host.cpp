@@ -6,8 +6,10 @@
 //   Boundary / Shape ctors (boundary.cpp:48-56, 123-187, 264-301, 391-393):
 //     inward plane normal from the face's edge vectors, offset, rotation and
 //     reflection matrices.
-//   Parallelepiped / TriangularPrism / Tetrahedron faces (subdomain.h:149-154,
-//     228-233, 305-309) and Subdomain::Subdomain (subdomain.cpp:41-71).
+//   Parallelepiped / TriangularPrism / Tetrahedron / Prism / Pyramid faces
+//     (subdomain.h:149-154, 228-233, 305-309, 398-409, 537-546), their
+//     volumes (subdomain.cpp:379-441) and Subdomain::Subdomain
+//     (subdomain.cpp:41-71).
 //   makePair (boundary.cpp:361-369, 524-550), emitter registry
 //     (subdomain.cpp:205-212, domain.cpp:88-102), Field::init column layout
 //     (field.cpp:25-45) and cell volumes (subdomain.cpp:269-349).
@@ -136,9 +138,24 @@ uint32_t prob_to_thr(double p) { return p >= 1.0 ? 0xFFFFFFFFu : uint32_t(p * 42
 struct FaceDef {
     Vec o, i, j;
     int shape;
+    int nfan = 0;                  // SHAPE_POLY: vertices after the first,
+    Vec fan[MCG_MAX_SIDES - 1];    // relative to o (Polygon<N>, boundary.cpp:197-216)
 };
 
-int faces_of(int cell, const Vec& o, const double* m, FaceDef* f, int& nface) {
+// Sides of a prism's or pyramid's N-gon base for a face count, 0 if out of range.
+int base_sides(int cell, int nface) {
+    const int n = cell == MCG_PRISM ? nface - 2 : cell == MCG_PYRAMID ? nface - 1 : 0;
+    return n >= 4 && n <= MCG_MAX_SIDES ? n : 0;
+}
+
+FaceDef polygon(const Vec& o, const Vec* v, int nv) {
+    FaceDef f{o, v[0], v[1], SHAPE_POLY};
+    f.nfan = nv;
+    for (int k = 0; k < nv; ++k) f.fan[k] = v[k];
+    return f;
+}
+
+int faces_of(int cell, int nf_in, const Vec& o, const double* m, FaceDef* f, int& nface) {
     Vec a = column(m, 0), b = column(m, 1), c = column(m, 2);
     switch (cell) {
         case MCG_PARALLELEPIPED:   // Bac Lef Bot Fro Rig Top
@@ -165,12 +182,74 @@ int faces_of(int cell, const Vec& o, const double* m, FaceDef* f, int& nface) {
             f[3] = {plus(o, a), minus(c, a), minus(b, a), SHAPE_TRI};
             nface = 4;
             return MCG_OK;
+        case MCG_PRISM:            // Bot Top Sid0..Sid(N-1) (subdomain.h:398-409, 462-479)
+        case MCG_PYRAMID: {        // Bot Sid0..Sid(N-1) (subdomain.h:537-546, 598-617)
+            const int n = base_sides(cell, nf_in);
+            if (!n) return set_error(MCG_EINVAL, "a prism or pyramid base needs 4 to 9 sides");
+            Vec v[MCG_MAX_SIDES];
+            for (int k = 0; k < n; ++k) v[k] = column(m, k);
+            Vec rev[MCG_MAX_SIDES - 1];
+            for (int k = 0; k < n - 1; ++k) rev[k] = v[n - 1 - k];
+            int q = 0;
+            f[q++] = polygon(o, v + 1, n - 1);                          // base, vertices 1..N-1
+            if (cell == MCG_PRISM) f[q++] = polygon(plus(o, a), rev, n - 1);   // top, reversed
+            for (int s = 0; s < n; ++s) {
+                // side s runs from vertex s to vertex s + 1 (vertex N is the origin's)
+                const Vec p = s == 0 ? mk(0.0, 0.0, 0.0) : v[s];
+                const Vec j = minus(s != n - 1 ? v[s + 1] : mk(0.0, 0.0, 0.0), p);
+                f[q++] = cell == MCG_PRISM ? FaceDef{plus(o, p), a, j, SHAPE_PAR}
+                                           : FaceDef{plus(o, p), minus(a, p), j, SHAPE_TRI};
+            }
+            nface = q;
+            return MCG_OK;
+        }
     }
     return set_error(MCG_EINVAL, "unknown cell kind");
 }
 
+// Volume pieces of a prism (div 2) or pyramid (div 6) as the reference's
+// PrismImpl / PyramidImpl::volume compute them (subdomain.cpp:385-394,
+// 417-426): term i is (m_i x m_(i+1)) . m_0 / div for i = 0..N-3, so term 0
+// is zero (m_0 is the axis) and the fan's last triangle is left out; their
+// drawPos picks term i and draws in the piece with edges (m_(i+1), m_(i+2),
+// m_0) (subdomain.cpp:401-409, 433-441).  Both are kept as written.
+void volume_terms(const double* m, int n, double div, double* w) {
+    const Vec a = column(m, 0);
+    for (int i = 0; i < n - 2; ++i) w[i] = dotp(crossp(column(m, i), column(m, i + 1)), a) / div;
+}
+
+// Piece table of an emitter: the pieces' edges and their cumulative,
+// normalised weights.  A weight that is negative only by rounding (the
+// first volume term, (m_0 x m_1) . m_0) counts as zero; pcum is exactly 1
+// from the last piece of nonzero weight on, so no later piece is drawn.
+int set_pieces(DevEmitter<double>& e, const Vec* pa, const Vec* pb, const double* w, int n) {
+    double total = 0.0, mag = 0.0;
+    for (int k = 0; k < n; ++k) mag += std::fabs(w[k]);
+    int last = -1;
+    for (int k = 0; k < n; ++k) {
+        if (w[k] < -1e-12 * mag) return set_error(MCG_EINVAL, "Volume too small, check vector order");
+        if (w[k] > 0.0) {
+            total += w[k];
+            last = k;
+        }
+    }
+    if (last < 0) return set_error(MCG_EINVAL, "Volume too small, check vector order");
+    double run = 0.0;
+    e.npiece = n;
+    for (int k = 0; k < n; ++k) {
+        run += std::max(w[k], 0.0);
+        e.pcum[k] = k >= last ? 1.0 : run / total;
+        for (int r = 0; r < 3; ++r) {
+            e.pa[k][r] = pa[k][r];
+            e.pb[k][r] = pb[k][r];
+        }
+    }
+    return MCG_OK;
+}
+
 double volume_of_cell(int cell, double vol, const int64_t shape[3], const int64_t idx[3]) {
     if (cell == MCG_PARALLELEPIPED) return vol / double(shape[0] * shape[1] * shape[2]);
+    if (cell == MCG_PRISM || cell == MCG_PYRAMID) return vol;   // subdomain.cpp:396-399, 428-431
     if (cell == MCG_TRIPRISM) {
         double s0 = double(shape[0]), s1 = double(shape[1]);
         double f0 = 1.0 - (double(idx[0]) / s0 + double(idx[1]) / s1);
@@ -349,26 +428,51 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
     std::vector<int32_t> peri_pair;
     for (int si = 0; si < ns; ++si) {
         const mcg_subdomain_spec& c = spec->sdom[si];
-        if (c.cell < MCG_PARALLELEPIPED || c.cell > MCG_TETRAHEDRON)
+        if (c.cell < MCG_PARALLELEPIPED || c.cell > MCG_PYRAMID)
             return set_error(MCG_EINVAL, "unknown cell kind");
-        const int want = c.cell == MCG_PARALLELEPIPED ? 6 : c.cell == MCG_TRIPRISM ? 5 : 4;
+        const bool ngon = c.cell == MCG_PRISM || c.cell == MCG_PYRAMID;
+        const int nside = ngon ? base_sides(c.cell, c.nface) : 3;
+        if (ngon && !nside) return set_error(MCG_EINVAL, "a prism or pyramid base needs 4 to 9 sides");
+        const int want = c.cell == MCG_PARALLELEPIPED ? 6 : c.cell == MCG_TRIPRISM ? 5
+                       : c.cell == MCG_TETRAHEDRON ? 4 : c.nface;
         if (c.nface != want) return set_error(MCG_EINVAL, "Incorrect number of temperatures");
-        const double det = determinant(c.mat);
-        const double vol = c.cell == MCG_PARALLELEPIPED ? det : c.cell == MCG_TRIPRISM ? det / 2.0 : det / 6.0;
+        // the matrix the cell's coordinates use: the edges, or for an N-gon
+        // cell (m_1, m_(N-1), m_0) (subdomain.cpp:379-383, 411-415)
+        double base[9];
+        if (ngon) {
+            const int cols[3] = {1, nside - 1, 0};
+            for (int k = 0; k < 3; ++k)
+                for (int r = 0; r < 3; ++r) base[3 * k + r] = c.mat[3 * cols[k] + r];
+        } else {
+            std::memcpy(base, c.mat, sizeof(base));
+        }
+        double vol;
+        if (ngon) {
+            double w[MCG_MAX_SIDES];
+            volume_terms(c.mat, nside, c.cell == MCG_PRISM ? 2.0 : 6.0, w);
+            vol = 0.0;
+            for (int k = 0; k < nside - 2; ++k) vol += w[k];
+        } else {
+            const double det = determinant(c.mat);
+            vol = c.cell == MCG_PARALLELEPIPED ? det : c.cell == MCG_TRIPRISM ? det / 2.0 : det / 6.0;
+        }
         if (!(vol >= DBL_MIN)) return set_error(MCG_EINVAL, "Volume too small, check vector order");
         sdom_vol[si] = vol;
         DevSdom<double> d{};
         const Vec o = mk(c.origin[0], c.origin[1], c.origin[2]);
         for (int k = 0; k < 3; ++k) d.o[k] = o[k];
-        invert(c.mat, d.inv);
-        // accumulation flag and grid shape (subdomain.cpp:41-71)
+        invert(base, d.inv);
+        // accumulation flag and grid shape (subdomain.cpp:41-71); an N-gon
+        // cell passes (-1, -1, -1) or (0, 0, 0) (subdomain.h:388-391)
+        int64_t div[3];
+        for (int k = 0; k < 3; ++k) div[k] = ngon ? (c.div[0] < 0 ? -1 : 0) : c.div[k];
         int positive = 0;
         bool negative = false;
         for (int k = 0; k < 3; ++k) {
-            negative |= c.div[k] < 0;
-            positive += c.div[k] > 0;
-            shapes[si][k] = std::max<int64_t>(c.div[k], 1);
-            d.divf[k] = double(c.div[k]);
+            negative |= div[k] < 0;
+            positive += div[k] > 0;
+            shapes[si][k] = std::max<int64_t>(div[k], 1);
+            d.divf[k] = double(div[k]);
         }
         if (negative) {
             d.accum = -2;
@@ -378,7 +482,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
         } else if (positive == 1) {
             int axis = 0;
             for (int k = 1; k < 3; ++k)
-                if (c.div[k] > c.div[axis]) axis = k;
+                if (div[k] > div[axis]) axis = k;
             d.accum = axis;
             s.has_1d = true;
         } else {
@@ -390,12 +494,12 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             d.max[k] = int32_t(negative ? 0 : shapes[si][k] - 1);
         }
         double colmax = 0.0;
-        for (int k = 0; k < 3; ++k) colmax = std::max(colmax, len(column(c.mat, k)));
+        for (int k = 0; k < 3; ++k) colmax = std::max(colmax, len(column(base, k)));
         d.eps = 100.0 * DBL_EPSILON * colmax;
         d.pore = -1;
-        FaceDef fd[6];
+        FaceDef fd[MCG_MAX_FACES];
         int nf = 0;
-        int rc = faces_of(c.cell, o, c.mat, fd, nf);
+        int rc = faces_of(c.cell, c.nface, o, c.mat, fd, nf);
         if (rc) return rc;
         d.pbeg = int32_t(s.planes.size());
         for (int f = 0; f < nf; ++f) {
@@ -406,9 +510,20 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             if (!emits && fs.temp != 0.0) return set_error(MCG_EINVAL, "Cannot specify temperature");
             DevPlane<double> P{};
             const Vec cr = crossp(fd[f].i, fd[f].j);
-            const double area = fd[f].shape == SHAPE_PAR ? len(cr) : len(cr) / 2.0;
-            if (area == 0.0) return set_error(MCG_EINVAL, "Shape not initialized");
             const Vec n = unit(cr);
+            double area = fd[f].shape == SHAPE_PAR ? len(cr) : len(cr) / 2.0;
+            if (fd[f].shape == SHAPE_POLY) {
+                // the fan's triangles, each of positive area and facing the
+                // polygon's way (boundary.cpp:197-216)
+                area = 0.0;
+                for (int k = 0; k + 1 < fd[f].nfan; ++k) {
+                    const Vec t = crossp(fd[f].fan[k], fd[f].fan[k + 1]);
+                    if (!(len(t) / 2.0 > DBL_MIN)) return set_error(MCG_EINVAL, "Area too small");
+                    if (!approx_vec(unit(t), n)) return set_error(MCG_EINVAL, "Normals are inconsistent");
+                    area += len(t) / 2.0;
+                }
+            }
+            if (area == 0.0) return set_error(MCG_EINVAL, "Shape not initialized");
             for (int k = 0; k < 3; ++k) P.n[k] = n[k];
             P.c = -dotp(n, fd[f].o);
             rotation_to(n, P.rot);
@@ -590,6 +705,21 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             e.shape = c.cell;
             e.sdom = si;
             e.sign = 0;
+            if (c.cell == MCG_PRISM || c.cell == MCG_PYRAMID) {
+                // piece i: edges (m_(i+1), m_(i+2), m_0), weight volume term i
+                const int n = base_sides(c.cell, c.nface);
+                double w[MCG_MAX_SIDES];
+                volume_terms(c.mat, n, c.cell == MCG_PRISM ? 2.0 : 6.0, w);
+                Vec pa[kMaxPieces], pb[kMaxPieces];
+                for (int k = 0; k < n - 2; ++k) {
+                    pa[k] = column(c.mat, k + 1);
+                    pb[k] = column(c.mat, k + 2);
+                }
+                const Vec a = column(c.mat, 0);
+                for (int r = 0; r < 3; ++r) e.mat[6 + r] = a[r];
+                const int rc = set_pieces(e, pa, pb, w, n - 2);
+                if (rc) return rc;
+            }
             s.emitters.push_back(e);
             s.emit_weight.push_back(wv);
         }
@@ -610,6 +740,12 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             e.shape = f.shape;
             e.sdom = si;
             e.sign = temp_now[pi] >= 0.0 ? 1 : 0;
+            if (f.shape == SHAPE_POLY) {   // the fan's triangles by area (boundary.cpp:238-251)
+                double w[kMaxPieces];
+                for (int k = 0; k + 1 < f.nfan; ++k) w[k] = len(crossp(f.fan[k], f.fan[k + 1])) / 2.0;
+                const int rc = set_pieces(e, f.fan, f.fan + 1, w, f.nfan - 1);
+                if (rc) return rc;
+            }
             s.emitters.push_back(e);
             // a face across the pore's axis has its disk cut out
             const double area = pk >= 0 && P.n[2] != 0.0

@@ -34,7 +34,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kChunk = 256;            // bundle ids claimed per wavefront grab
 constexpr int kMaxRows = 4;            // nonzero amount entries per step
-constexpr int kMaxFaces = 6;           // faces of the largest cell (the parallelepiped)
+constexpr int kMaxFaces = 6;           // faces an FP32 step keeps in registers (the parallelepiped's)
 constexpr size_t kLdsTallyCap = 48 * 1024;
 // Emission inside a pored box: position redraw k uses Philox blocks
 // kPosRedrawBlock + 2k (two coordinates) and + 2k + 1 (the third); a pore
@@ -200,8 +200,18 @@ __device__ __forceinline__ F draw_path(const dev::Event& ev, const uint4& h, con
     return d;
 }
 
+// The piece of a polygon face or N-gon cell a uniform selects: the first k
+// with u < pcum[k] (pcum reaches exactly 1 at the last piece drawn).
+template <typename F>
+__device__ __forceinline__ int pick_piece(const DevEmitter<F>& E, F u) {
+    int k = 0;
+    while (k + 1 < E.npiece && !(u < E.pcum[k])) ++k;
+    return k;
+}
+
 // Emitter::emit with the wall / volume draws, then drawScatNext
-// (problem.cpp:386-399, boundary.cpp:378-431, subdomain.cpp:255-377).
+// (problem.cpp:386-399, boundary.cpp:378-431 and 244-251 for a polygon,
+// subdomain.cpp:255-441).
 template <typename F>
 __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* mfp_tab,
                      int64_t n, Lane<F>& L) {
@@ -218,12 +228,20 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
     if (!E.volume) {
         const F r1 = dev::Real<F>::uniform(b2.x, b3.x), r2 = dev::Real<F>::uniform(b2.y, b3.y);
         F a = r1, b = r2;
-        if (E.shape == SHAPE_TRI && !(r1 + r2 < F(1))) {
+        if (E.shape != SHAPE_PAR && !(r1 + r2 < F(1))) {
             a = F(1) - r1;
             b = F(1) - r2;
         }
-        L.pos = dev::V3<F>{E.o[0] + (a * E.si[0] + b * E.sj[0]), E.o[1] + (a * E.si[1] + b * E.sj[1]),
-                           E.o[2] + (a * E.si[2] + b * E.sj[2])};
+        F si[3] = {E.si[0], E.si[1], E.si[2]}, sj[3] = {E.sj[0], E.sj[1], E.sj[2]};
+        if (E.npiece) {   // a polygon: one triangle of its fan, by area
+            const int k = pick_piece(E, dev::Real<F>::uniform(b2.w, b3.w));
+            for (int r = 0; r < 3; ++r) {
+                si[r] = E.pa[k][r];
+                sj[r] = E.pb[k][r];
+            }
+        }
+        L.pos = dev::V3<F>{E.o[0] + (a * si[0] + b * sj[0]), E.o[1] + (a * si[1] + b * sj[1]),
+                           E.o[2] + (a * si[2] + b * sj[2])};
         const int pk = K.npore > 0 ? K.sdoms[E.sdom].pore : -1;
         if (pk >= 0) {   // a wall across the pore (a box face): redraw inside its disk
             const DevPore<F>& Q = K.pores[pk];
@@ -245,7 +263,7 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
             c0 = F(1) - c0;
             c1 = F(1) - c1;
         }
-        if (E.shape == MCG_TETRAHEDRON) {
+        if (E.shape == MCG_TETRAHEDRON || E.shape == MCG_PYRAMID) {
             if (c1 + c2 > F(1)) {
                 const F t = c2;
                 c2 = F(1) - c0 - c1;
@@ -256,7 +274,16 @@ __device__ void emit(const KParams<F>& K, const uint64_t* flux_alias, const F* m
                 c0 = F(1) - c1 - t;
             }
         }
-        const dev::V3<F> m = dev::matvec(E.mat, dev::V3<F>{c0, c1, c2});
+        F M[9];
+        for (int r = 0; r < 9; ++r) M[r] = E.mat[r];
+        if (E.npiece) {   // a prism / pyramid: one piece, by its volume term
+            const int k = pick_piece(E, dev::Real<F>::uniform(b2.w, b3.w));
+            for (int r = 0; r < 3; ++r) {
+                M[r] = E.pa[k][r];
+                M[3 + r] = E.pb[k][r];
+            }
+        }
+        const dev::V3<F> m = dev::matvec(M, dev::V3<F>{c0, c1, c2});
         L.pos = dev::V3<F>{E.o[0] + m.x, E.o[1] + m.y, E.o[2] + m.z};
         const int pk = K.npore > 0 ? K.sdoms[E.sdom].pore : -1;
         if (pk >= 0) {   // uniform in the box minus the pore: redraw inside it
@@ -736,6 +763,19 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
                     rate[i] = nd;
                 }
             }
+            // faces beyond the first kMaxFaces (prisms and pyramids with
+            // large bases) are tested again after the move
+            for (int b = S.pbeg + kMaxFaces; b < S.pend; ++b) {
+                const DevPlane<F>& P = V.planes[b];
+                const F nd = dev::dot3(P.n, L.dir);
+                const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
+                                                                            __builtin_fmaf(P.n[0], L.pos.x, P.c)));
+                const F t = -d * __builtin_amdgcn_rcpf(nd);
+                if (nd < F(0) && t < dmin) {
+                    dmin = t;
+                    hit = b;
+                }
+            }
         }
     } else {
         (void)dist;
@@ -809,6 +849,10 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             outside = false;
 #pragma unroll
             for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -S.eps;
+            for (int b = S.pbeg + kMaxFaces; b < S.pend; ++b) {
+                const DevPlane<F>& P = V.planes[b];
+                outside |= dev::dot3(P.n, L.pos) + P.c < -S.eps;
+            }
         }
         if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
     } else if (S.box) {
@@ -1296,6 +1340,11 @@ void convert_emitter(const DevEmitter<double>& s, DevEmitter<F>& d) {
     for (int i = 0; i < 3; ++i) { d.o[i] = F(s.o[i]); d.si[i] = F(s.si[i]); d.sj[i] = F(s.sj[i]); d.grad[i] = F(s.grad[i]); }
     for (int i = 0; i < 9; ++i) { d.mat[i] = F(s.mat[i]); d.rot[i] = F(s.rot[i]); }
     d.volume = s.volume; d.shape = s.shape; d.sdom = s.sdom; d.sign = s.sign;
+    d.npiece = s.npiece;
+    for (int k = 0; k < kMaxPieces; ++k) {
+        d.pcum[k] = F(s.pcum[k]);
+        for (int r = 0; r < 3; ++r) { d.pa[k][r] = F(s.pa[k][r]); d.pb[k][r] = F(s.pb[k][r]); }
+    }
 }
 
 template <typename F>

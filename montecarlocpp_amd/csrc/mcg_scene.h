@@ -30,10 +30,14 @@ enum : int {
     W_POS1 = 8,
     W_POS2 = 9,
     W_POS3 = 10,
+    W_PIECE = 11,
     W_REDRAW = 16
 };
 
-enum ShapeKind : int32_t { SHAPE_PAR = 0, SHAPE_TRI = 1 };
+enum ShapeKind : int32_t { SHAPE_PAR = 0, SHAPE_TRI = 1, SHAPE_POLY = 2 };
+
+// Triangles of a polygon face's fan, or pieces of a prism / pyramid's volume
+constexpr int kMaxPieces = MCG_MAX_SIDES - 2;
 
 template <typename F>
 struct DevPlane {
@@ -73,13 +77,21 @@ template <typename F>
 struct DevEmitter {
     F o[3];
     F si[3], sj[3];   // wall shape vectors
-    F mat[9];         // volume: edge matrix
+    F mat[9];         // volume: edge matrix (prism / pyramid: column 2 only)
     F rot[9];         // wall: rotMatrix(normal); volume: rotMatrix(gradT)
     F grad[3];
     int32_t volume;
-    int32_t shape;    // wall: SHAPE_PAR / SHAPE_TRI; volume: mcg_cell_kind
+    int32_t shape;    // wall: ShapeKind; volume: mcg_cell_kind
     int32_t sdom;
     int32_t sign;     // wall: T >= 0
+    // SHAPE_POLY walls and prism / pyramid volumes: piece k is drawn when
+    // u < pcum[k] for the first such k (pcum is cumulative, normalised, and
+    // exactly 1 from the last piece of nonzero weight on); a polygon's piece
+    // is the triangle (pa, pb), a volume's the triangular prism or
+    // tetrahedron with edges (pa, pb, mat column 2)
+    int32_t npiece;
+    F pa[kMaxPieces][3], pb[kMaxPieces][3];
+    F pcum[kMaxPieces];
 };
 
 template <typename F>

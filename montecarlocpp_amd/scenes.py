@@ -31,18 +31,33 @@ class Domain:
 
     # -- construction -------------------------------------------------------
     def add_cell(self, cell, origin, mat_cols, div, faces, grad_t=(0.0, 0.0, 0.0)):
-        """Add a convex cell; ``mat_cols`` is a 3x3 array whose *columns* are
-        the edge vectors (the reference's ``mat``).  Returns its index."""
-        nface = abi.NFACE[cell]
+        """Add a convex cell; ``mat_cols`` is a 3 x K array whose *columns*
+        are the reference's ``mat``: the three edges, or for a prism / pyramid
+        with an N-gon base its N columns (axis or apex, then the base's
+        vertices 1..N-1, relative to ``origin``).  An N-gon cell's ``div`` is
+        the reference's single ``long div`` (negative: no tally, otherwise one
+        cell).  Returns the cell's index."""
+        m = np.asarray(mat_cols, dtype=np.float64)
+        if m.ndim != 2 or m.shape[0] != 3:
+            raise ValueError("mat_cols must have 3 rows")
+        ngon = cell in (abi.PRISM, abi.PYRAMID)
+        ncol = m.shape[1]
+        if ngon and not 4 <= ncol <= abi.MAX_SIDES:
+            raise ValueError(f"a prism or pyramid base needs 4 to {abi.MAX_SIDES} sides")
+        if not ngon and ncol != 3:
+            raise ValueError("this cell kind takes 3 edge columns")
+        nface = abi.nface(cell, ncol)
         if len(faces) != nface:
             raise ValueError(f"cell kind {cell} needs {nface} faces")
         s = abi.SubdomainSpec()
         s.cell = cell
         s.nface = nface
-        m = np.asarray(mat_cols, dtype=np.float64)
         s.origin[:] = [float(v) for v in origin]
-        s.mat[:] = [float(v) for v in m.T.reshape(-1)]    # column-major
-        s.div[:] = [int(v) for v in div]
+        s.mat[:3 * ncol] = [float(v) for v in m.T.reshape(-1)]    # column-major
+        if ngon:
+            s.div[:] = [int(div), 0, 0]
+        else:
+            s.div[:] = [int(v) for v in div]
         s.grad_t[:] = [float(v) for v in grad_t]
         for i, (k, t, p) in enumerate(faces):
             s.face[i].kind = k

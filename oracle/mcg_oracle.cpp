@@ -16,7 +16,8 @@
  *   Subdomain ctor / isInside / coord  subdomain.cpp:41-71, 108-116, 148-159
  *   Subdomain::advect                  subdomain.cpp:161-192
  *   emitter registry                   subdomain.cpp:199-212, domain.cpp:88-102
- *   EmitSubdomain, cell draws/volumes  subdomain.cpp:232-377
+ *   EmitSubdomain, cell draws/volumes  subdomain.cpp:232-441
+ *   Polygon faces                      boundary.cpp:190-251
  *   Field::init / accumulate           field.cpp:25-45, 92-220
  *   FieldProblem ctor / solve          problem.cpp:315-342, 370-445
  *   tally kinds                        problem.cpp:468-648
@@ -253,6 +254,7 @@ enum : int {
     W_POS1 = 8,     // emission position draws
     W_POS2 = 9,
     W_POS3 = 10,
+    W_PIECE = 11,   // emission: triangle of a polygon's fan, piece of an N-gon cell
     W_REDRAW = 16,  // free path redraw k: words 16 + 2k (high), 17 + 2k (low)
     B_POSRE = 64    // emission position redraw k: blocks 64 + 2k (high) and 65 + 2k (low)
 };
@@ -453,7 +455,45 @@ int build_material(const mcg_material_spec* s, Material& m) {
 }
 
 /* ----------------------------------------------------------- geometry */
-enum ShapeKind { SHAPE_PAR = 0, SHAPE_TRI = 1 };
+enum ShapeKind { SHAPE_PAR = 0, SHAPE_TRI = 1, SHAPE_POLY = 2 };
+
+// Discrete draw over a polygon's fan triangles or an N-gon cell's volume
+// pieces: weights as the reference's DiscreteDist receives them, drawn by
+// inverse CDF (the first k with u < cum[k]).
+struct Pieces {
+    int n = 0;
+    V3 a[MCG_MAX_SIDES - 2], b[MCG_MAX_SIDES - 2];
+    double cum[MCG_MAX_SIDES - 2];
+    int pick(double u) const {
+        int k = 0;
+        while (k + 1 < n && !(u < cum[k])) ++k;
+        return k;
+    }
+};
+
+// Normalised cumulative weights; a weight below zero only by rounding is
+// zero, and cum is exactly 1 from the last positive weight on.
+int make_pieces(Pieces& p, const V3* a, const V3* b, const double* w, int n) {
+    double mag = 0.0, total = 0.0, run = 0.0;
+    int last = -1;
+    for (int k = 0; k < n; ++k) mag += std::fabs(w[k]);
+    for (int k = 0; k < n; ++k) {
+        if (w[k] < -1e-12 * mag) return fail(MCG_EINVAL, "Volume too small, check vector order");
+        if (w[k] > 0.0) {
+            total += w[k];
+            last = k;
+        }
+    }
+    if (last < 0) return fail(MCG_EINVAL, "Volume too small, check vector order");
+    p.n = n;
+    for (int k = 0; k < n; ++k) {
+        run += std::max(w[k], 0.0);
+        p.cum[k] = k >= last ? 1.0 : run / total;
+        p.a[k] = a[k];
+        p.b[k] = b[k];
+    }
+    return MCG_OK;
+}
 
 struct Plane {
     V3 n;           // inward unit normal
@@ -464,6 +504,7 @@ struct Plane {
     V3 o;           // face origin
     int shape;
     V3 si, sj;      // shape vectors
+    Pieces fan;     // SHAPE_POLY: the fan's triangles (boundary.cpp:197-216, 244-251)
     double area;
     double T;       // after pairing
     bool emitter;   // registered at cell construction (subdomain.cpp:205-212)
@@ -489,6 +530,8 @@ struct Sdom {
     double vol;
     V3 gradT;
     M3 rotG;
+    Pieces vpieces; // prism / pyramid volume pieces (subdomain.cpp:385-441)
+    V3 axis;        // their column 0: the prism's axis, the pyramid's apex
     int pbeg, pend;
     int64_t colOff;
     int64_t stride[3];
@@ -520,33 +563,73 @@ struct Domain {
 struct FaceGeo {
     V3 o, i, j;
     int shape;
+    std::vector<V3> fan;   // SHAPE_POLY: vertices after the first, relative to o
 };
+
+// Sides N of an N-gon cell's base from its face count (0 when not in 4..9).
+int ngon_sides(const mcg_subdomain_spec& s) {
+    const int n = s.cell == MCG_PRISM ? s.nface - 2 : s.cell == MCG_PYRAMID ? s.nface - 1 : 0;
+    return n >= 4 && n <= MCG_MAX_SIDES ? n : 0;
+}
+
+V3 mat_col(const mcg_subdomain_spec& s, int k) {
+    return v3(s.mat[3 * k], s.mat[3 * k + 1], s.mat[3 * k + 2]);
+}
+
+// PrismImpl / PyramidImpl::volume (subdomain.cpp:385-394, 417-426), as
+// written: term i = (m_i x m_(i+1)) . m_0 / div for i = 0..N-3.
+void ngon_volume_terms(const mcg_subdomain_spec& s, int n, double div, double* w) {
+    const V3 a = mat_col(s, 0);
+    for (int i = 0; i < n - 2; ++i) w[i] = dot(cross(mat_col(s, i), mat_col(s, i + 1)), a) / div;
+}
 
 int cell_faces(const mcg_subdomain_spec& s, const M3& mat, V3 o, std::vector<FaceGeo>& f) {
     V3 m0 = col(mat, 0), m1 = col(mat, 1), m2 = col(mat, 2);
     f.clear();
     switch (s.cell) {
         case MCG_PARALLELEPIPED:  // subdomain.h:149-154
-            f.push_back({o, m1, m2, SHAPE_PAR});
-            f.push_back({o, m2, m0, SHAPE_PAR});
-            f.push_back({o, m0, m1, SHAPE_PAR});
-            f.push_back({add(o, m0), m2, m1, SHAPE_PAR});
-            f.push_back({add(o, m1), m0, m2, SHAPE_PAR});
-            f.push_back({add(o, m2), m1, m0, SHAPE_PAR});
+            f.push_back({o, m1, m2, SHAPE_PAR, {}});
+            f.push_back({o, m2, m0, SHAPE_PAR, {}});
+            f.push_back({o, m0, m1, SHAPE_PAR, {}});
+            f.push_back({add(o, m0), m2, m1, SHAPE_PAR, {}});
+            f.push_back({add(o, m1), m0, m2, SHAPE_PAR, {}});
+            f.push_back({add(o, m2), m1, m0, SHAPE_PAR, {}});
             return MCG_OK;
         case MCG_TRIPRISM:  // subdomain.h:228-233
-            f.push_back({o, m1, m2, SHAPE_PAR});
-            f.push_back({o, m2, m0, SHAPE_PAR});
-            f.push_back({o, m0, m1, SHAPE_TRI});
-            f.push_back({add(o, m0), m2, sub(m1, m0), SHAPE_PAR});
-            f.push_back({add(o, m2), m1, m0, SHAPE_TRI});
+            f.push_back({o, m1, m2, SHAPE_PAR, {}});
+            f.push_back({o, m2, m0, SHAPE_PAR, {}});
+            f.push_back({o, m0, m1, SHAPE_TRI, {}});
+            f.push_back({add(o, m0), m2, sub(m1, m0), SHAPE_PAR, {}});
+            f.push_back({add(o, m2), m1, m0, SHAPE_TRI, {}});
             return MCG_OK;
         case MCG_TETRAHEDRON:  // subdomain.h:305-309
-            f.push_back({o, m1, m2, SHAPE_TRI});
-            f.push_back({o, m2, m0, SHAPE_TRI});
-            f.push_back({o, m0, m1, SHAPE_TRI});
-            f.push_back({add(o, m0), sub(m2, m0), sub(m1, m0), SHAPE_TRI});
+            f.push_back({o, m1, m2, SHAPE_TRI, {}});
+            f.push_back({o, m2, m0, SHAPE_TRI, {}});
+            f.push_back({o, m0, m1, SHAPE_TRI, {}});
+            f.push_back({add(o, m0), sub(m2, m0), sub(m1, m0), SHAPE_TRI, {}});
             return MCG_OK;
+        case MCG_PRISM:     // subdomain.h:398-409 (Bot, Top), 462-479 (sides)
+        case MCG_PYRAMID: { // subdomain.h:537-546 (Bot), 598-617 (sides)
+            const int n = ngon_sides(s);
+            if (!n) return fail(MCG_EINVAL, "a prism or pyramid base needs 4 to 9 sides");
+            std::vector<V3> bot, top;
+            for (int k = 1; k < n; ++k) bot.push_back(mat_col(s, k));
+            for (int k = n - 1; k >= 1; --k) top.push_back(mat_col(s, k));
+            const V3 ax = mat_col(s, 0);
+            f.push_back({o, bot[0], bot[1], SHAPE_POLY, bot});
+            if (s.cell == MCG_PRISM) f.push_back({add(o, ax), top[0], top[1], SHAPE_POLY, top});
+            for (int ind = 0; ind < n; ++ind) {
+                V3 p = v3(0.0, 0.0, 0.0);
+                if (ind != 0) p = add(p, mat_col(s, ind));
+                V3 j = scale(-1.0, p);
+                if (ind != n - 1) j = add(j, mat_col(s, ind + 1));
+                if (s.cell == MCG_PRISM)
+                    f.push_back({add(o, p), ax, j, SHAPE_PAR, {}});
+                else
+                    f.push_back({add(o, p), sub(ax, p), j, SHAPE_TRI, {}});
+            }
+            return MCG_OK;
+        }
     }
     return fail(MCG_EINVAL, "unknown cell kind");
 }
@@ -555,6 +638,9 @@ double cell_volume(const Sdom& s, const int64_t idx[3]) {
     switch (s.cell) {
         case MCG_PARALLELEPIPED:  // subdomain.cpp:269-273
             return s.vol / double(s.shape[0] * s.shape[1] * s.shape[2]);
+        case MCG_PRISM:           // subdomain.cpp:396-399
+        case MCG_PYRAMID:         // subdomain.cpp:428-431
+            return s.vol;
         case MCG_TRIPRISM: {  // subdomain.cpp:283-307
             double s0 = double(s.shape[0]), s1 = double(s.shape[1]);
             double f0 = 1.0 - (double(idx[0]) / s0 + double(idx[1]) / s1);
@@ -602,22 +688,43 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
         S.cell = s.cell;
         S.o = v3(s.origin[0], s.origin[1], s.origin[2]);
         std::memcpy(S.mat.m, s.mat, sizeof(S.mat.m));
-        double dt = det(S.mat);
-        int nface_expect = s.cell == MCG_PARALLELEPIPED ? 6 : s.cell == MCG_TRIPRISM ? 5 : 4;
-        if (s.cell < 0 || s.cell > 2) return fail(MCG_EINVAL, "unknown cell kind");
+        if (s.cell < MCG_PARALLELEPIPED || s.cell > MCG_PYRAMID) return fail(MCG_EINVAL, "unknown cell kind");
+        const bool ngon = s.cell == MCG_PRISM || s.cell == MCG_PYRAMID;
+        const int nside = ngon_sides(s);
+        if (ngon && !nside) return fail(MCG_EINVAL, "a prism or pyramid base needs 4 to 9 sides");
+        int nface_expect = s.cell == MCG_PARALLELEPIPED ? 6 : s.cell == MCG_TRIPRISM ? 5
+                         : s.cell == MCG_TETRAHEDRON ? 4 : s.nface;
         if (s.nface != nface_expect) return fail(MCG_EINVAL, "Incorrect number of temperatures");
-        S.vol = s.cell == MCG_PARALLELEPIPED ? dt : s.cell == MCG_TRIPRISM ? dt / 2.0 : dt / 6.0;
+        if (ngon) {
+            // the cell's coordinate matrix (m_1, m_(N-1), m_0) (subdomain.cpp:379-383, 411-415)
+            const V3 c[3] = {mat_col(s, 1), mat_col(s, nside - 1), mat_col(s, 0)};
+            for (int k = 0; k < 3; ++k) {
+                S.mat.m[3 * k] = c[k].x;
+                S.mat.m[3 * k + 1] = c[k].y;
+                S.mat.m[3 * k + 2] = c[k].z;
+            }
+            double w[MCG_MAX_SIDES];
+            ngon_volume_terms(s, nside, s.cell == MCG_PRISM ? 2.0 : 6.0, w);
+            S.vol = 0.0;
+            for (int k = 0; k < nside - 2; ++k) S.vol += w[k];
+        } else {
+            double dt = det(S.mat);
+            S.vol = s.cell == MCG_PARALLELEPIPED ? dt : s.cell == MCG_TRIPRISM ? dt / 2.0 : dt / 6.0;
+        }
         if (!(S.vol >= DBL_MIN)) return fail(MCG_EINVAL, "Volume too small, check vector order");
         S.inv = inverse(S.mat);
-        // subdomain.cpp:41-71
+        // subdomain.cpp:41-71; an N-gon cell's div is (-1, -1, -1) or
+        // (0, 0, 0) by the sign of its one div (subdomain.h:388-391)
         bool anyNeg = false;
         int npos = 0;
+        int64_t dv[3];
+        for (int k = 0; k < 3; ++k) dv[k] = ngon ? (s.div[0] < 0 ? -1 : 0) : s.div[k];
         for (int k = 0; k < 3; ++k) {
-            S.div[k] = s.div[k];
-            S.shape[k] = std::max<int64_t>(s.div[k], 1);
+            S.div[k] = dv[k];
+            S.shape[k] = std::max<int64_t>(dv[k], 1);
             S.max[k] = S.shape[k] - 1;
-            if (s.div[k] < 0) anyNeg = true;
-            if (s.div[k] > 0) ++npos;
+            if (dv[k] < 0) anyNeg = true;
+            if (dv[k] > 0) ++npos;
         }
         if (anyNeg) {
             for (int k = 0; k < 3; ++k) S.shape[k] = S.max[k] = 0;
@@ -627,10 +734,26 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
         } else if (npos == 1) {
             int dir = 0;
             for (int k = 1; k < 3; ++k)
-                if (s.div[k] > s.div[dir]) dir = k;
+                if (dv[k] > dv[dir]) dir = k;
             S.accum = dir;
         } else {
             S.accum = npos + 1;
+        }
+        if (ngon) {
+            // drawPos: piece i has edges (m_(i+1), m_(i+2), m_0) and weight
+            // volume term i (subdomain.cpp:401-409, 433-441)
+            double w[MCG_MAX_SIDES];
+            ngon_volume_terms(s, nside, s.cell == MCG_PRISM ? 2.0 : 6.0, w);
+            V3 pa[MCG_MAX_SIDES], pb[MCG_MAX_SIDES];
+            for (int k = 0; k < nside - 2; ++k) {
+                pa[k] = mat_col(s, k + 1);
+                pb[k] = mat_col(s, k + 2);
+            }
+            S.axis = mat_col(s, 0);
+            if (s.grad_t[0] != 0.0 || s.grad_t[1] != 0.0 || s.grad_t[2] != 0.0) {
+                int rc = make_pieces(S.vpieces, pa, pb, w, nside - 2);
+                if (rc) return rc;
+            }
         }
         double mx = 0.0;
         for (int k = 0; k < 3; ++k) mx = std::max(mx, norm(col(S.mat, k)));
@@ -652,8 +775,24 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
             P.sj = fg[f].j;
             V3 cr = cross(P.si, P.sj);
             P.area = P.shape == SHAPE_PAR ? norm(cr) : norm(cr) / 2.0;
-            if (P.area == 0.0) return fail(MCG_EINVAL, "Shape not initialized");
             P.n = normalized(cr);
+            if (P.shape == SHAPE_POLY) {
+                // Polygon<N>: fan triangles of positive area, all facing the
+                // polygon's normal; area is their sum (boundary.cpp:197-216, 238-242)
+                const std::vector<V3>& v = fg[f].fan;
+                double w[MCG_MAX_SIDES];
+                P.area = 0.0;
+                for (size_t k = 0; k + 1 < v.size(); ++k) {
+                    const V3 t = cross(v[k], v[k + 1]);
+                    w[k] = norm(t) / 2.0;
+                    if (!(w[k] > DBL_MIN)) return fail(MCG_EINVAL, "Area too small");
+                    if (!vec_approx(normalized(t), P.n)) return fail(MCG_EINVAL, "Normals are inconsistent");
+                    P.area += w[k];
+                }
+                int rc = make_pieces(P.fan, v.data(), v.data() + 1, w, int(v.size()) - 1);
+                if (rc) return rc;
+            }
+            if (P.area == 0.0) return fail(MCG_EINVAL, "Shape not initialized");
             P.c = -dot(P.n, P.o);
             P.T = fs.temp;
             if (fs.kind < MCG_SPEC || fs.kind > MCG_PARTIAL)
@@ -938,11 +1077,17 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
     if (!e.volume) {
         const Plane& P = D.planes[e.index];
         double r1 = g.u53(W_POS1), r2 = g.u53(W_POS2);
+        V3 si = P.si, sj = P.sj;
+        if (P.shape == SHAPE_POLY) {   // Polygon<N>::drawPos (boundary.cpp:244-251)
+            const int k = P.fan.pick(g.u53(W_PIECE));
+            si = P.fan.a[k];
+            sj = P.fan.b[k];
+        }
         V3 local;
         if (P.shape == SHAPE_PAR || r1 + r2 < 1.0)
-            local = add(scale(r1, P.si), scale(r2, P.sj));
+            local = add(scale(r1, si), scale(r2, sj));
         else
-            local = add(scale(1.0 - r1, P.si), scale(1.0 - r2, P.sj));
+            local = add(scale(1.0 - r1, si), scale(1.0 - r2, sj));
         pos = add(P.o, local);
         if (D.sdoms[P.sdom].pore >= 0) {   // a wall across the pore: redraw inside its disk
             const Pore& Q = D.pores[D.sdoms[P.sdom].pore];
@@ -959,13 +1104,13 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
     } else {
         const Sdom& S = D.sdoms[e.index];
         double c[3] = {g.u53(W_POS1), g.u53(W_POS2), g.u53(W_POS3)};
-        if (S.cell == MCG_TRIPRISM || S.cell == MCG_TETRAHEDRON) {
+        if (S.cell != MCG_PARALLELEPIPED) {
             if (c[0] + c[1] > 1.0) {
                 c[0] = 1.0 - c[0];
                 c[1] = 1.0 - c[1];
             }
         }
-        if (S.cell == MCG_TETRAHEDRON) {
+        if (S.cell == MCG_TETRAHEDRON || S.cell == MCG_PYRAMID) {
             if (c[1] + c[2] > 1.0) {
                 double tmp = c[2];
                 c[2] = 1.0 - c[0] - c[1];
@@ -976,7 +1121,18 @@ Phonon emit(const Material& M, const Domain& D, const Emitter& e, EventRng& g, i
                 c[0] = 1.0 - c[1] - tmp;
             }
         }
-        pos = add(S.o, mul(S.mat, v3(c[0], c[1], c[2])));
+        M3 local = S.mat;
+        if (S.cell == MCG_PRISM || S.cell == MCG_PYRAMID) {
+            // Prism / PyramidImpl::drawPos (subdomain.cpp:401-409, 433-441)
+            const int k = S.vpieces.pick(g.u53(W_PIECE));
+            const V3 cols[3] = {S.vpieces.a[k], S.vpieces.b[k], S.axis};
+            for (int q = 0; q < 3; ++q) {
+                local.m[3 * q] = cols[q].x;
+                local.m[3 * q + 1] = cols[q].y;
+                local.m[3 * q + 2] = cols[q].z;
+            }
+        }
+        pos = add(S.o, mul(local, v3(c[0], c[1], c[2])));
         if (S.pore >= 0) {   // uniform in the box minus the pore: redraw inside it
             const Pore& P = D.pores[S.pore];
             for (int k = 0; k < kMaxPosRedraw; ++k) {

@@ -1,0 +1,165 @@
+"""Prism and pyramid cells with an N-gon base (subdomain.h:364-630, Polygon<N>
+boundary.cpp:190-251) in the checker and the engine's host builder.
+
+What pins them: the polygon faces' areas against the shoelace formula, their
+emission against uniformity over the polygon (share and centroid per fan
+triangle), and the reference's volume loop (subdomain.cpp:385-394, 417-426)
+against its closed form -- it starts at the axis column, so it is the base's
+area without the fan's last triangle, times the height (prism) or a third of
+it (pyramid) -- with volume emission confined to the pieces its drawPos
+selects (subdomain.cpp:401-409, 433-441)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from montecarlocpp_amd import abi, engine, materials, scenes
+from oracle import oracle
+from tests import ngon_scenes as ng
+
+U = ng.U
+
+
+@pytest.fixture(scope="module")
+def mat():
+    return materials.si_like(300.0)
+
+
+def test_prism_weights_and_volume(mat):
+    """Volume source 2 V_ref |gradT| first, then the Isot base and top of
+    area A (shoelace); V_ref = (A - area(v0, v3, v4)) h."""
+    d = ng.prism()
+    A = ng.shoelace(ng.PENTAGON)
+    p = ng.PENTAGON
+    v_ref = (A - ng.tri_area(p[0], p[3], p[4])) * ng.PRISM_AXIS[2]
+    for lib in (oracle, engine):
+        info, w, n = lib.plan(mat, d, scenes.problem(1000))
+        np.testing.assert_allclose(w, [2 * v_ref * 2e6, A * 1.0, A * 1.0], rtol=1e-13)
+        assert info.cols == 1
+    np.testing.assert_allclose(oracle.cell_volumes(d), [v_ref], rtol=1e-13)
+
+
+def test_pyramid_weights_and_volume(mat):
+    """Pyramid: V_ref = (A - area(v0, v2, v3)) h / 3; the last side is an Isot
+    triangle of area |(apex - v3) x (v0 - v3)| / 2."""
+    d = ng.pyramid(grad=(1e6, 0.0, 0.0))
+    q = ng.QUAD
+    A = ng.shoelace(q)
+    v_ref = (A - ng.tri_area(q[0], q[2], q[3])) * ng.APEX[2] / 3
+    v3 = np.array([q[3][0], q[3][1], 0.0])
+    side = 0.5 * np.linalg.norm(np.cross(ng.APEX - v3, -v3))
+    for lib in (oracle, engine):
+        info, w, n = lib.plan(mat, d, scenes.problem(1000))
+        np.testing.assert_allclose(w, [2 * v_ref * 1e6, A * 1.0, side * 1.0], rtol=1e-13)
+    np.testing.assert_allclose(oracle.cell_volumes(d), [v_ref], rtol=1e-13)
+
+
+def test_no_tally_with_negative_div(mat):
+    for lib in (oracle, engine):
+        info, _, _ = lib.plan(mat, ng.prism(div=-1), scenes.problem(100))
+        assert info.cols == 0
+
+
+def _in_triangle(p, a, b, c, tol=1e-12):
+    """Points within the counter-clockwise triangle (a, b, c), up to tol U^2
+    (a negative tol asks for the interior at that margin)."""
+    def s(u, v, w):
+        return (v[0] - u[0]) * (w[:, 1] - u[1]) - (v[1] - u[1]) * (w[:, 0] - u[0])
+    d1, d2, d3 = s(a, b, p), s(b, c, p), s(c, a, p)
+    m = -tol * U * U
+    return (d1 >= m) & (d2 >= m) & (d3 >= m)
+
+
+def test_polygon_emission_uniform(mat):
+    """A pentagon face emits uniformly: every start lies on it, each fan
+    triangle gets its share of the area and its starts average to its
+    centroid."""
+    d = ng.prism(grad=(0.0, 0.0, 0.0), t_top=0.0)
+    n = 60000
+    e = oracle.sample_emission(mat, d, scenes.problem(n, seed=9), n)
+    assert np.all(e[:, 2] == 0.0)
+    p = ng.PENTAGON
+    A = ng.shoelace(p)
+    seen = np.zeros(n, bool)
+    for k in range(1, 4):
+        inside = _in_triangle(e[:, :2], p[0], p[k], p[k + 1])
+        seen |= inside
+        share = ng.tri_area(p[0], p[k], p[k + 1]) / A
+        cnt = np.count_nonzero(inside)
+        assert abs(cnt - n * share) < 5 * np.sqrt(n * share * (1 - share)), k
+        cen = (p[0] + p[k] + p[k + 1]) / 3
+        np.testing.assert_allclose(e[inside, :2].mean(axis=0), cen, atol=0.02 * U)
+    assert seen.all()
+
+
+def test_prism_volume_emission_as_the_reference_draws(mat):
+    """The volume source draws piece i (edges m_(i+1), m_(i+2), axis) with
+    weight term i; term 0 is zero, so no start falls in the prism over the
+    fan's first triangle (v0, v1, v2)."""
+    d = ng.prism(t_bot=0.0, t_top=0.0, grad=(0.0, 0.0, 2e6))
+    n = 60000
+    e = oracle.sample_emission(mat, d, scenes.problem(n, seed=4), n)
+    p = ng.PENTAGON
+    h = ng.PRISM_AXIS[2]
+    z = e[:, 2]
+    assert np.all((z >= 0) & (z <= h))
+    # project each start along the axis back onto the base plane
+    base = e[:, :2] - np.outer(z / h, ng.PRISM_AXIS[:2])
+    w = [ng.tri_area(p[0], p[1], p[2]), ng.tri_area(p[0], p[2], p[3])]   # terms 1, 2
+    pieces = [(p[0], p[2], p[3]), (p[0], p[3], p[4])]
+    total = 0
+    for (a, b, c), wk in zip(pieces, w):
+        cnt = np.count_nonzero(_in_triangle(base, a, b, c))
+        share = wk / sum(w)
+        assert abs(cnt - n * share) < 5 * np.sqrt(n * share * (1 - share))
+        total += cnt
+    assert total == n
+    assert not np.any(_in_triangle(base, p[0], p[1], p[2], tol=-1e-6))
+
+
+def test_pyramid_volume_emission_inside_pieces(mat):
+    """A 4-gon pyramid's volume terms are (0, term 1): every start is in the
+    tetrahedron (v0, v2, v3, apex), uniformly."""
+    q = ng.QUAD
+    dd = scenes.Domain("pyr")
+    dd.add_cell(abi.PYRAMID, (0.0, 0.0, 0.0), ng.ngon_mat(ng.APEX, q), 0,
+                [scenes.face(abi.DIFF)] * 5, grad_t=(1e6, 0.0, 0.0))
+    n = 40000
+    e = oracle.sample_emission(mat, dd, scenes.problem(n, seed=6), n)
+    # one piece: the tetrahedron (v0, v2, v3, apex); barycentric coordinates
+    v = [np.array([*q[2], 0.0]), np.array([*q[3], 0.0]), ng.APEX]
+    M = np.array(v).T
+    lam = np.linalg.solve(M, e[:, :3].T).T
+    assert np.all(lam >= -1e-9) and np.all(lam.sum(axis=1) <= 1 + 1e-9)
+    # uniform in it: the mean is the centroid
+    np.testing.assert_allclose(e[:, :3].mean(axis=0), (v[0] + v[1] + v[2]) / 4, atol=0.01 * U)
+
+
+def _raw_prism(nface, base):
+    d = ng.prism(base)
+    d.cells[0].nface = nface
+    return d
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("three_sides", "needs 4 to 9 sides"),
+    ("inconsistent", "Normals are inconsistent"),
+    ("clockwise", "Volume too small"),
+])
+def test_ngon_errors(mat, case, msg):
+    if case == "three_sides":
+        d = _raw_prism(5, ng.PENTAGON)
+    elif case == "inconsistent":
+        base = np.array([[0.0, 0.0], [2.0, 0.0], [2.0, 2.0], [3.0, 1.0]]) * U
+        d = ng.prism(base)
+    else:
+        d = ng.prism(ng.PENTAGON[[0, 4, 3, 2, 1]])
+    for lib in (oracle, engine):
+        with pytest.raises(Exception, match=msg):
+            lib.plan(mat, d, scenes.problem(100))
+
+
+def test_mirror_struct_sizes():
+    """The Python mirror of mcg_subdomain_spec holds MAX_SIDES columns and
+    MAX_FACES faces."""
+    assert C.sizeof(abi.SubdomainSpec) == 4 + 4 + 24 + 8 * 3 * abi.MAX_SIDES + 24 + 24 + 24 * abi.MAX_FACES
