@@ -33,10 +33,11 @@
  * energy pdf; this has the distribution of the reference's two-level draw
  * (material.cpp:51-75): P(w, p) = pdf(w, p) / sum(pdf).
  *
- * Parity status of each part (how it is pinned) is listed in the tests; the
- * cell-volume formulas of the triangular prism and the tetrahedron
- * (subdomain.cpp:283-349) are restated but only exercised by
- * tests/test_oracle_geometry.py on small cases.
+ * Each part is pinned by an independent check in tests/: closed forms,
+ * brute force on small cases, known-answer vectors, distribution tests and
+ * invariants (the cell-volume formulas of the triangular prism and the
+ * tetrahedron, subdomain.cpp:283-349, against brute-force cell shares in
+ * test_geometry.py; the N-gon cells in test_ngon.py).
  */
 #include "../include/mcg.h"
 // The engine's portable FP64 log and sin/cos(pi r): the oracle uses them in

@@ -163,3 +163,30 @@ def test_mirror_struct_sizes():
     """The Python mirror of mcg_subdomain_spec holds MAX_SIDES columns and
     MAX_FACES faces."""
     assert C.sizeof(abi.SubdomainSpec) == 4 + 4 + 24 + 8 * 3 * abi.MAX_SIDES + 24 + 24 + 24 * abi.MAX_FACES
+
+
+@pytest.mark.parametrize("cell", [abi.TRIPRISM, abi.TETRAHEDRON])
+def test_simplex_volume_draw_uniform(mat, cell):
+    """The folding of TriangularPrism / TetrahedronImpl::drawPos
+    (subdomain.cpp:309-377) maps the unit cube onto the cell uniformly: in
+    the cell's own coordinates every start lies inside, and the share with
+    coordinate 0 below 1/2 is 1 - (1/2)^2 (prism) or 1 - (1/2)^3
+    (tetrahedron), with the mean at the centroid."""
+    M = np.array([[2e-7, 0.4e-7, 0.1e-7], [0.3e-7, 1.5e-7, 0.2e-7], [0.0, 0.2e-7, 1e-7]]).T
+    d = scenes.Domain("simplex")
+    nf = abi.NFACE[cell]
+    d.add_cell(cell, (0.0, 0.0, 0.0), M, (0, 0, 0), [scenes.face(abi.DIFF)] * nf, grad_t=(1e6, 0.0, 0.0))
+    n = 60000
+    e = oracle.sample_emission(mat, d, scenes.problem(n, seed=12), n)
+    c = np.linalg.solve(M, e[:, :3].T).T
+    assert np.all(c >= -1e-12) and np.all(c <= 1 + 1e-12)
+    if cell == abi.TRIPRISM:
+        assert np.all(c[:, 0] + c[:, 1] <= 1 + 1e-12)
+        share, cen = 1 - 0.5 ** 2, np.array([1 / 3, 1 / 3, 1 / 2])
+    else:
+        assert np.all(c.sum(axis=1) <= 1 + 1e-12)
+        share, cen = 1 - 0.5 ** 3, np.array([0.25, 0.25, 0.25])
+    for k in range(3 if cell == abi.TETRAHEDRON else 2):
+        cnt = np.count_nonzero(c[:, k] < 0.5)
+        assert abs(cnt - n * share) < 5 * np.sqrt(n * share * (1 - share)), k
+    np.testing.assert_allclose(c.mean(axis=0), cen, atol=0.01)
