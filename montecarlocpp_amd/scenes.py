@@ -209,6 +209,12 @@ def tube(dim, div, dT):
     return d
 
 
+def octet(dim, div, dT):
+    """OctetDomain (domain.cpp:591-1225): see :mod:`montecarlocpp_amd.octet`."""
+    from .octet import octet as build
+    return build(dim, div, dT)
+
+
 def porous_film(dim=(300e-9, 300e-9, 100e-9), radius=100e-9, div=(64, 64, 0), dT=1.0,
                 pore=abi.DIFF, walls=abi.DIFF, specularity=0.0):
     """Periodic porous film: a unit cell periodic in x (with the volumetric

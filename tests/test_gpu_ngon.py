@@ -1,6 +1,7 @@
 """Prisms and pyramids with N-gon bases on the GPU: a slanted pentagonal
-prism with a volume source, a 4-gon pyramid and a regular 9-gon prism (11
-faces: more than an FP32 step keeps in registers).  Deterministic mode equals
+prism with a volume source, a 4-gon pyramid, a regular 9-gon prism (11
+faces: more than an FP32 step keeps in registers) and the 42-cell octet-truss
+unit (OctetDomain) built from them.  Deterministic mode equals
 the FP64 checker bundle by bundle; FP32 flight runs every bundle, loses at
 most one in a thousand and agrees with FP64 within 4 standard errors over
 seeds.  A rectangular 4-gon prism against the parallelepiped of the same box
@@ -10,7 +11,7 @@ is taken into account."""
 import numpy as np
 import pytest
 
-from montecarlocpp_amd import abi, engine, materials, scenes
+from montecarlocpp_amd import abi, engine, materials, octet, scenes
 from oracle import oracle
 from tests import ngon_scenes as ng
 from tests.helpers import assert_counts_equal, rownorm_rel
@@ -26,6 +27,7 @@ SCENES = {
     "pentagon_prism": lambda: ng.prism(),
     "quad_pyramid": lambda: ng.pyramid(grad=(0.0, 1e6, 0.0)),
     "nonagon_prism": _nonagon,
+    "octet": lambda: octet.octet((6e-7, 1e-7, 1e-7, 1e-8), (4, 2, 2, 1), 1.0),
 }
 
 
