@@ -135,6 +135,18 @@ int main(int argc, const char* argv[]) {
             long n1, n3;
             argss >> d0 >> d1 >> d3 >> n1 >> n3;
             dom.reset(new TubeDomain({d0, d1, d1, d3}, {0, n1, n1, n3}, 1e6 * d0));
+        } else if (domStr == "octet") {
+            // main.cpp:376-386: dim0..dim3, div0..div3, dT
+            std::array<double, 4> dim{};
+            std::array<long, 4> div{};
+            double dT = 0.;
+            for (double& v : dim) argss >> v;
+            for (long& v : div) argss >> v;
+            argss >> dT;
+            if (!argss.fail())
+                for (double v : dim)
+                    if (!(v > 0.)) throw mcg::Error(MCG_EINVAL, "Dimensions must be positive");
+            if (!argss.fail()) dom.reset(new OctetDomain(dim, div, dT));
         } else if (domStr == "porous") {
             double side, thick, radius;
             long div;

@@ -870,6 +870,262 @@ public:
     }
 };
 
+using Peri4 = PeriBoundary<Polygon<4>>;
+template <typename... S>
+using Sides = std::tuple<S...>;
+
+inline Matrix3d cols3(const Vector3d& a, const Vector3d& b, const Vector3d& c) {
+    Matrix3d m;
+    const Vector3d* v[3] = {&a, &b, &c};
+    for (int k = 0; k < 3; ++k)
+        for (int r = 0; r < 3; ++r) m(r, k) = (*v[k])(r);
+    return m;
+}
+inline Matrix3Xd cols4(const Vector3d& a, const Vector3d& b, const Vector3d& c, const Vector3d& d) {
+    Matrix3Xd m(3, 4);
+    const Vector3d* v[4] = {&a, &b, &c, &d};
+    for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r) m(r, k) = (*v[k])(r);
+    return m;
+}
+
+// OctetDomain (domain.h:299-350, domain.cpp:591-1225): one periodic unit of
+// an octet nanotruss.  Nodes of 4-gon prisms, pyramids, triangular prisms and
+// boxes, and ten struts (cells 16-25, the only sources and tallies), joined
+// by 71 interface pairs and closed by 5 periodic pairs mapping x to -x.  The
+// cells' geometry is the reference's (a table that has to be alike).
+class OctetDomain : public Domain {
+public:
+    using SdomCont = std::tuple<
+        Prism<Inter, Peri4, Sides<Inter, Diff, Spec, Diff>>,      // 00
+        Prism<Diff, Inter, Sides<Inter, Diff, Spec, Inter>>,      // 01
+        Prism<Diff, Diff, Sides<Inter, Inter, Spec, Spec>>,       // 02
+        Prism<Inter, Peri4, Sides<Diff, Inter, Inter, Diff>>,     // 03
+        Prism<Inter, Inter, Sides<Diff, Inter, Inter, Inter>>,    // 04
+        Parallelepiped<Inter, Spec, Diff, Diff, Inter, Inter>,    // 05
+        Prism<Spec, Inter, Sides<PeriP, Diff, Inter, Diff>>,      // 06
+        Prism<Inter, Diff, Sides<PeriP, Diff, Inter, Inter>>,     // 07
+        Prism<Diff, Diff, Sides<PeriP, Inter, Inter, Inter>>,     // 08
+        Prism<Diff, Diff, Sides<Inter, Inter, Inter, Inter>>,     // 09
+        Prism<Inter, Diff, Sides<Inter, Inter, Diff, Inter>>,     // 10
+        Prism<Spec, Inter, Sides<Diff, Inter, Diff, Inter>>,      // 11
+        TriangularPrism<Diff, Inter, Inter, Spec, Inter>,         // 12
+        Pyramid<Inter, Sides<Diff, Inter, Spec, Inter>>,          // 13
+        TriangularPrism<Inter, Inter, Diff, Spec, Inter>,         // 14
+        Prism<Diff, Diff, Sides<Inter, Inter, Spec, Spec>>,       // 15
+        Parallelepiped<Diff, Spec, Inter, Diff, Inter, Inter>,    // 16
+        Parallelepiped<Inter, Inter, Inter, Diff, Diff, Inter>,   // 17
+        Parallelepiped<Inter, Diff, Inter, Inter, Diff, Inter>,   // 18
+        Parallelepiped<Diff, Inter, Inter, Inter, Diff, Inter>,   // 19
+        Parallelepiped<Diff, Spec, Inter, Diff, Inter, Inter>,    // 20
+        Parallelepiped<Diff, Spec, Inter, Diff, Inter, Inter>,    // 21
+        Parallelepiped<Diff, Inter, Inter, Inter, Diff, Inter>,   // 22
+        Parallelepiped<Inter, Diff, Inter, Inter, Diff, Inter>,   // 23
+        Parallelepiped<Inter, Inter, Inter, Diff, Diff, Inter>,   // 24
+        Parallelepiped<Diff, Spec, Inter, Diff, Inter, Inter>,    // 25
+        Prism<Diff, Diff, Sides<Inter, Inter, Spec, Spec>>,       // 26
+        TriangularPrism<Inter, Inter, Diff, Spec, Inter>,         // 27
+        Pyramid<Inter, Sides<Diff, Inter, Spec, Inter>>,          // 28
+        TriangularPrism<Diff, Inter, Inter, Spec, Inter>,         // 29
+        Prism<Spec, Inter, Sides<Diff, Inter, Diff, Inter>>,      // 30
+        Prism<Inter, Diff, Sides<Inter, Inter, Diff, Inter>>,     // 31
+        Prism<Diff, Diff, Sides<Inter, Inter, Inter, Inter>>,     // 32
+        Prism<Diff, Diff, Sides<PeriP, Inter, Inter, Inter>>,     // 33
+        Prism<Inter, Diff, Sides<PeriP, Diff, Inter, Inter>>,     // 34
+        Prism<Spec, Inter, Sides<PeriP, Diff, Inter, Diff>>,      // 35
+        Parallelepiped<Inter, Spec, Diff, Diff, Inter, Inter>,    // 36
+        Prism<Inter, Inter, Sides<Diff, Inter, Inter, Inter>>,    // 37
+        Prism<Inter, Peri4, Sides<Diff, Inter, Inter, Diff>>,     // 38
+        Prism<Diff, Diff, Sides<Inter, Inter, Spec, Spec>>,       // 39
+        Prism<Diff, Inter, Sides<Inter, Diff, Spec, Inter>>,      // 40
+        Prism<Inter, Peri4, Sides<Inter, Diff, Spec, Diff>>>;     // 41
+    template <size_t I>
+    using Cell = typename std::tuple_element<I, SdomCont>::type;
+
+private:
+    std::array<double, 4> dim_;
+    std::array<long, 4> div_;
+    double dT_;
+    std::array<std::unique_ptr<Subdomain>, 42> cells_;
+
+    template <size_t I, typename M, typename D>
+    void make(const Vector3d& o, const M& mat, const D& div, const Vector3d& g = Vector3d::Zero()) {
+        cells_[I].reset(new Cell<I>(o, mat, div, g));
+    }
+
+public:
+    template <size_t I>
+    Cell<I>& sdom() { return static_cast<Cell<I>&>(*cells_[I]); }
+
+    OctetDomain(const std::array<double, 4>& dim, const std::array<long, 4>& div, double dT)
+        : dim_(dim), div_(div), dT_(dT) {
+        const double s = dim[0], t = dim[3], r2 = std::sqrt(2.0);
+        const double pi = 3.141592653589793;          // constants.h:17
+        const double a = pi / 5. * (dim[1] + dim[2]) - (4. - pi) / 5. * t, b = a / 4.;
+        if (!(b > t) || !(a > (r2 + 1.) * (b + t)) || !(s / 2. > a + (r2 + 1.) * (b + t) + t))
+            throw mcg::Error(MCG_EINVAL, "Invalid dimensions");
+        const long n0 = div[0], n1 = div[1], n2 = div[2], n3 = div[3];
+        const long no = -1;                           // noDiv(0)
+        const Vector3l none(-1, -1, -1);              // noDiv
+        const double gx = dT / (s - 2. * a - 2. * (r2 + 1.) * b - 2. * (2. + r2) * t);
+        const Vector3d g(gx, 0., -gx);
+        using V = Vector3d;
+        // horizontal top: 00-05
+        {
+            const V c1(r2 * t, 0., 0.);
+            const V c2(s / 4. - (2. + r2) / 2. * b - (2. - r2) / 2. * t, s / 4. - (2. + r2) / 2. * (b + t), 0.);
+            const V c3(s / 4. - (2. + r2) / 2. * b - t, s / 4. - (2. + r2) / 2. * b - t, 0.);
+            make<0>(V((r2 + 1.) * b + t, b + t, -a), cols4(V(0., 0., a), c1, c2, c3), no);
+            make<1>(V((r2 + 1.) * b + t, b + t, -a - t), cols4(V(0., 0., t), c1, c2, c3), no);
+            make<2>(V(b + t, b + t, -a - t),
+                    cols4(V(0., 0., t), V(r2 * b, 0., 0.),
+                          V(s / 4. - (2. - r2) / 2. * b - t, s / 4. - (2. + r2) / 2. * b - t, 0.),
+                          V(s / 4. - b - t, s / 4. - b - t, 0.)), no);
+            const V d1((r2 + 1.) * t, 0., 0.), d2((r2 + 1.) * t, t, 0.), d3(t, t, 0.);
+            make<3>(V((r2 + 1.) * b, b, -a), cols4(V(0., 0., a), d1, d2, d3), no);
+            make<4>(V((r2 + 1.) * b, b, -a - t), cols4(V(0., 0., t), d1, d2, d3), no);
+            make<5>(V(b, b, -a - t), cols3(V(0., 0., t), V(r2 * b, 0., 0.), V(t, t, 0.)), none);
+        }
+        // node top: 06-15
+        {
+            const V c1(r2 * t, 0., 0.);
+            const V c2((r2 + 1.) / 2. * b + (r2 + 1.) * t, 0., -(r2 + 1.) / 2. * b - t);
+            const V c3((r2 + 1.) / 2. * b + (2. + r2) / 2. * t, 0., -(r2 + 1.) / 2. * b - (2. + r2) / 2. * t);
+            make<6>(V(a, 0., 0.), cols4(V(0., b, 0.), c1, c2, c3), no);
+            make<7>(V(a, b, 0.), cols4(V(0., t, 0.), c1, c2, c3), no);
+            make<8>(V((r2 + 1.) * (b + t), b, 0.),
+                    cols4(V(0., t, 0.), V(a - (r2 + 1.) * (b + t), 0., 0.),
+                          V(a - (r2 + 1.) / 2. * b - t / r2, 0., -(r2 + 1.) / 2. * b - (2. + r2) / 2. * t),
+                          V(0., 0., -a - t)), no);
+            make<9>(V(b + t, b, -a - t),
+                    cols4(V(0., t, 0.), V(r2 * (b + t), 0., 0.),
+                          V((r2 - 1.) / 2. * b + t / r2, 0., -(r2 + 1.) / 2. * b - t / r2), V(0., 0., -b)), no);
+            const V e1((r2 - 1.) / 2. * b + t / r2, 0., -(r2 - 1.) / 2. * b - t / r2);
+            const V e2((r2 - 1.) / 2. * b, 0., -(r2 - 1.) / 2. * b - r2 * t), e3(0., 0., -r2 * t);
+            make<10>(V(b + t, b, -a - b - t), cols4(V(0., t, 0.), e1, e2, e3), no);
+            make<11>(V(b + t, 0., -a - b - t), cols4(V(0., b, 0.), e1, e2, e3), no);
+            make<12>(V(b + t, b, -a - b), cols3(V(0., t, 0.), V(-t, 0., 0.), V(0., 0., b - t)), none);
+            make<13>(V(b + t, b, -a - b - t), cols4(V(-t, 0., t), V(0., 0., t), V(0., t, t), V(0., t, 0.)), no);
+            make<14>(V(b + t, b, -a - b - (r2 + 1.) * t), cols3(V(0., t, 0.), V(-t, 0., t), V(0., 0., r2 * t)),
+                     none);
+            make<15>(V(b + t, 0., -a - b - (r2 + 1.) * t),
+                     cols4(V(0., 0., r2 * t), V(0., b, 0.), V(-t, b, t), V(-b - t, 0., b + t)), no);
+        }
+        // struts: 16-20 (top) and 21-25 (bottom)
+        {
+            const double W = (r2 + 1.) / 2. * b + (2. + r2) / 2. * t, K = s / 4. - a / 2. - W, h = t / r2;
+            const V up(-K, 0., K), dn(K, 0., -K);
+            make<16>(V(s / 4. + a / 2., 0., -s / 4. + a / 2.), cols3(V(h, 0., h), V(0., b, 0.), up),
+                     Vector3l(n3, n2, n0), g);
+            make<17>(V(s / 4. + a / 2., b, -s / 4. + a / 2.), cols3(V(h, 0., h), V(0., t, 0.), up),
+                     Vector3l(n3, n3, n0), g);
+            make<18>(V(s / 4. - a / 2., b, -s / 4. - a / 2.), cols3(V(a, 0., a), V(0., t, 0.), up),
+                     Vector3l(n1, n3, n0), g);
+            make<19>(V(s / 4. - a / 2. - h, b, -s / 4. - a / 2. - h), cols3(V(h, 0., h), V(0., t, 0.), up),
+                     Vector3l(n3, n3, n0), g);
+            make<20>(V(s / 4. - a / 2. - h, 0., -s / 4. - a / 2. - h), cols3(V(h, 0., h), V(0., b, 0.), up),
+                     Vector3l(n3, n2, n0), g);
+            make<21>(V(s / 4. + a / 2. + h, 0., -s / 4. + a / 2. + h), cols3(V(-h, 0., -h), V(0., b, 0.), dn),
+                     Vector3l(n3, n2, n0), g);
+            make<22>(V(s / 4. + a / 2. + h, b, -s / 4. + a / 2. + h), cols3(V(-h, 0., -h), V(0., t, 0.), dn),
+                     Vector3l(n3, n3, n0), g);
+            make<23>(V(s / 4. + a / 2., b, -s / 4. + a / 2.), cols3(V(-a, 0., -a), V(0., t, 0.), dn),
+                     Vector3l(n1, n3, n0), g);
+            make<24>(V(s / 4. - a / 2., b, -s / 4. - a / 2.), cols3(V(-h, 0., -h), V(0., t, 0.), dn),
+                     Vector3l(n3, n3, n0), g);
+            make<25>(V(s / 4. - a / 2., 0., -s / 4. - a / 2.), cols3(V(-h, 0., -h), V(0., b, 0.), dn),
+                     Vector3l(n3, n2, n0), g);
+        }
+        // node bottom: 26-35
+        {
+            const double z0 = -s / 2. + a, x0 = s / 2. - b - t;
+            make<26>(V(x0, 0., z0 + b + (r2 + 1.) * t),
+                     cols4(V(0., 0., -r2 * t), V(0., b, 0.), V(t, b, -t), V(b + t, 0., -b - t)), no);
+            make<27>(V(x0, b, z0 + b + (r2 + 1.) * t), cols3(V(0., t, 0.), V(t, 0., -t), V(0., 0., -r2 * t)), none);
+            make<28>(V(x0, b, z0 + b + t), cols4(V(t, 0., -t), V(0., 0., -t), V(0., t, -t), V(0., t, 0.)), no);
+            make<29>(V(x0, b, z0 + b), cols3(V(0., t, 0.), V(t, 0., 0.), V(0., 0., -b + t)), none);
+            const V e1(-(r2 - 1.) / 2. * b - t / r2, 0., (r2 - 1.) / 2. * b + t / r2);
+            const V e2(-(r2 - 1.) / 2. * b, 0., (r2 - 1.) / 2. * b + r2 * t), e3(0., 0., r2 * t);
+            make<30>(V(x0, 0., z0 + b + t), cols4(V(0., b, 0.), e1, e2, e3), no);
+            make<31>(V(x0, b, z0 + b + t), cols4(V(0., t, 0.), e1, e2, e3), no);
+            make<32>(V(x0, b, z0 + t),
+                     cols4(V(0., t, 0.), V(-r2 * (b + t), 0., 0.),
+                           V(-(r2 - 1.) / 2. * b - t / r2, 0., (r2 + 1.) / 2. * b + t / r2), V(0., 0., b)), no);
+            make<33>(V(s / 2. - (r2 + 1.) * (b + t), b, -s / 2.),
+                     cols4(V(0., t, 0.), V(-a + (r2 + 1.) * (b + t), 0., 0.),
+                           V(-a + (r2 + 1.) / 2. * b + t / r2, 0., (r2 + 1.) / 2. * b + (2. + r2) / 2. * t),
+                           V(0., 0., a + t)), no);
+            const V c1(-r2 * t, 0., 0.);
+            const V c2(-(r2 + 1.) / 2. * b - (r2 + 1.) * t, 0., (r2 + 1.) / 2. * b + t);
+            const V c3(-(r2 + 1.) / 2. * b - (2. + r2) / 2. * t, 0., (r2 + 1.) / 2. * b + (2. + r2) / 2. * t);
+            make<34>(V(s / 2. - a, b, -s / 2.), cols4(V(0., t, 0.), c1, c2, c3), no);
+            make<35>(V(s / 2. - a, 0., -s / 2.), cols4(V(0., b, 0.), c1, c2, c3), no);
+        }
+        // horizontal bottom: 36-41
+        {
+            const double z0 = -s / 2. + a, x1 = s / 2. - (r2 + 1.) * b;
+            make<36>(V(s / 2. - b, b, z0 + t), cols3(V(0., 0., -t), V(-r2 * b, 0., 0.), V(-t, t, 0.)), none);
+            const V d1(-(r2 + 1.) * t, 0., 0.), d2(-(r2 + 1.) * t, t, 0.), d3(-t, t, 0.);
+            make<37>(V(x1, b, z0 + t), cols4(V(0., 0., -t), d1, d2, d3), no);
+            make<38>(V(x1, b, z0), cols4(V(0., 0., -a), d1, d2, d3), no);
+            make<39>(V(s / 2. - b - t, b + t, z0 + t),
+                     cols4(V(0., 0., -t), V(-r2 * b, 0., 0.),
+                           V(-s / 4. + (2. - r2) / 2. * b + t, s / 4. - (2. + r2) / 2. * b - t, 0.),
+                           V(-s / 4. + b + t, s / 4. - b - t, 0.)), no);
+            const V c1(-r2 * t, 0., 0.);
+            const V c2(-s / 4. + (2. + r2) / 2. * b + (2. - r2) / 2. * t, s / 4. - (2. + r2) / 2. * (b + t), 0.);
+            const V c3(-s / 4. + (2. + r2) / 2. * b + t, s / 4. - (2. + r2) / 2. * b - t, 0.);
+            make<40>(V(x1 - t, b + t, z0 + t), cols4(V(0., 0., -t), c1, c2, c3), no);
+            make<41>(V(x1 - t, b + t, z0), cols4(V(0., 0., -a), c1, c2, c3), no);
+        }
+        init();
+    }
+    OctetDomain(const OctetDomain&) = delete;
+    std::string info() const override {
+        std::ostringstream ss;
+        ss << "OctetDomain " << this << "\n  dim: [" << dim_[0] << " " << dim_[1] << " " << dim_[2] << " "
+           << dim_[3] << "]\n  div: [" << div_[0] << " " << div_[1] << " " << div_[2] << " " << div_[3]
+           << "]\n  dT:  " << dT_;
+        return ss.str();
+    }
+
+private:
+    // interface and periodic pairs (domain.cpp:595-687), then the cells in order
+    void init() {
+#define MCG_IP(A, FA, B, FB) makePair(sdom<A>().template bdry<FA>(), sdom<B>().template bdry<FB>())
+        MCG_IP(0, 0, 1, 1); MCG_IP(1, 5, 2, 3); MCG_IP(0, 2, 3, 4); MCG_IP(1, 2, 4, 4);
+        MCG_IP(2, 2, 5, 5); MCG_IP(3, 0, 4, 1); MCG_IP(4, 5, 5, 4);
+        MCG_IP(3, 3, 8, 5); MCG_IP(4, 3, 8, 5); MCG_IP(4, 0, 9, 2); MCG_IP(5, 0, 9, 2); MCG_IP(5, 0, 12, 4);
+        MCG_IP(6, 1, 7, 0); MCG_IP(7, 5, 8, 3); MCG_IP(9, 4, 10, 2); MCG_IP(10, 0, 11, 1);
+        MCG_IP(9, 5, 12, 1); MCG_IP(9, 5, 13, 0); MCG_IP(10, 5, 14, 1); MCG_IP(11, 5, 15, 2);
+        MCG_IP(12, 2, 13, 2); MCG_IP(13, 4, 14, 4); MCG_IP(14, 0, 15, 3);
+        MCG_IP(6, 4, 16, 5); MCG_IP(7, 4, 17, 5); MCG_IP(8, 4, 18, 5); MCG_IP(9, 3, 18, 5);
+        MCG_IP(10, 3, 19, 5); MCG_IP(11, 3, 20, 5);
+        MCG_IP(16, 4, 17, 1); MCG_IP(17, 0, 18, 3); MCG_IP(18, 0, 19, 3); MCG_IP(19, 1, 20, 4);
+        MCG_IP(16, 2, 21, 2); MCG_IP(17, 2, 22, 2); MCG_IP(18, 2, 23, 2); MCG_IP(19, 2, 24, 2);
+        MCG_IP(20, 2, 25, 2);
+        MCG_IP(21, 4, 22, 1); MCG_IP(22, 3, 23, 0); MCG_IP(23, 3, 24, 0); MCG_IP(24, 1, 25, 4);
+        MCG_IP(21, 5, 30, 3); MCG_IP(22, 5, 31, 3); MCG_IP(23, 5, 32, 3); MCG_IP(23, 5, 33, 4);
+        MCG_IP(24, 5, 34, 4); MCG_IP(25, 5, 35, 4);
+        MCG_IP(26, 3, 27, 0); MCG_IP(27, 4, 28, 4); MCG_IP(28, 2, 29, 2); MCG_IP(26, 2, 30, 5);
+        MCG_IP(27, 1, 31, 5); MCG_IP(28, 0, 32, 5); MCG_IP(29, 1, 32, 5); MCG_IP(30, 1, 31, 0);
+        MCG_IP(31, 2, 32, 4); MCG_IP(33, 3, 34, 5); MCG_IP(34, 0, 35, 1);
+        MCG_IP(29, 4, 36, 0); MCG_IP(32, 2, 36, 0); MCG_IP(32, 2, 37, 0); MCG_IP(33, 5, 37, 3);
+        MCG_IP(33, 5, 38, 3);
+        MCG_IP(36, 4, 37, 5); MCG_IP(37, 1, 38, 0); MCG_IP(36, 5, 39, 2); MCG_IP(37, 4, 40, 2);
+        MCG_IP(38, 4, 41, 2); MCG_IP(39, 3, 40, 5); MCG_IP(40, 1, 41, 0);
+#undef MCG_IP
+        const Vector3d transl(dim_[0] / 2., 0., -dim_[0] / 2.);
+        const Matrix3d rot = diag3(-1., 1., 1.);
+        makePair(sdom<0>().bdry<1>(), sdom<41>().bdry<1>(), transl, rot);
+        makePair(sdom<3>().bdry<1>(), sdom<38>().bdry<1>(), transl, rot);
+        makePair(sdom<6>().bdry<2>(), sdom<35>().bdry<2>(), transl, rot);
+        makePair(sdom<7>().bdry<2>(), sdom<34>().bdry<2>(), transl, rot);
+        makePair(sdom<8>().bdry<2>(), sdom<33>().bdry<2>(), transl, rot);
+        for (auto& c : cells_) addSdom(c.get());
+    }
+};
+
 // Periodic porous film (an extension: the reference's boundaries are planes).
 // One box cell periodic in x, with the volumetric gradient source -dT/dim0
 // of FilmDomain (domain.cpp:188-199), and in y; walls at z = 0 and z = dim2;

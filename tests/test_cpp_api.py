@@ -72,6 +72,8 @@ def deckdir(tmp_path):
     (["tee", 1e-7, 5e-8, 3], lambda: scenes.tee((1e-7,) * 4 + (5e-8,), (3, 3, 3, 3, 0), 3e6 * 1e-7)),
     (["tube", 2e-7, 1e-7, 5e-8, 4, 4],
      lambda: scenes.tube((2e-7, 1e-7, 1e-7, 5e-8), (0, 4, 4, 4), 1e6 * 2e-7)),
+    (["octet", 6e-7, 1e-7, 1e-7, 1e-8, 4, 2, 2, 1, 1.5],
+     lambda: scenes.octet((6e-7, 1e-7, 1e-7, 1e-8), (4, 2, 2, 1), 1.5)),
 ])
 def test_mirror_builds_the_same_scene(deckdir, args, py):
     got = dump(deckdir, args)
@@ -119,6 +121,7 @@ def parse_solution(out):
     ["grey", 300, "bulk", 2e-7, 8, "cumflux", 10000, 4, 9, 0, 1],
     ["silicon", 300, "porous", 3e-7, 1e-7, 1e-7, 16, "multi", 10000, 30, 0, 1],
     ["silicon", 300, "tube", 2e-7, 1e-7, 5e-8, 4, 4, "multi", 10000, 30, 0, 1],
+    ["silicon", 300, "octet", 6e-7, 1e-7, 1e-7, 1e-8, 4, 2, 2, 1, 1.0, "multi", 10000, 30, 0, 1],
 ])
 def test_driver_matches_checker(deckdir, deck):
     from oracle import oracle
@@ -139,6 +142,9 @@ def test_driver_matches_checker(deckdir, deck):
         dom = scenes.tube((deck[3], deck[4], deck[4], deck[5]), (0, deck[6], deck[6], deck[7]),
                           1e6 * deck[3])
         tally, nemit, maxscat, size = abi.MULTI, deck[9], deck[10], 0
+    elif kind == "octet":
+        dom = scenes.octet(deck[3:7], deck[7:11], deck[11])
+        tally, nemit, maxscat, size = abi.MULTI, deck[13], deck[14], 0
     elif kind == "porous":
         dom = scenes.porous_film((deck[3], deck[3], deck[4]), deck[5], (deck[6], deck[6], 0))
         tally, nemit, maxscat, size = abi.MULTI, deck[8], deck[9], 0
