@@ -1079,8 +1079,24 @@ public:
             make<41>(V(x1 - t, b + t, z0), cols4(V(0., 0., -a), c1, c2, c3), no);
         }
         init();
+        weigh();
     }
     OctetDomain(const OctetDomain&) = delete;
+
+    // OctetDomain::average (domain.cpp:1252-1280): one column, the mean of the
+    // tallies over the struts' (cells 16-25) first layer along their third
+    // axis, weighted by cell volume.
+    ArrayXXd average(const ArrayXXd& data) const override {
+        ArrayXXd out(data.rows(), 1);
+        double wsum = 0.0;
+        for (double w : weight_) wsum += w;
+        for (long r = 0; r < data.rows(); ++r) {
+            double acc = 0.0;
+            for (long c = 0; c < data.cols(); ++c) acc += data(r, c) * weight_[size_t(c)];
+            out(r, 0) = acc / wsum;
+        }
+        return out;
+    }
     std::string info() const override {
         std::ostringstream ss;
         ss << "OctetDomain " << this << "\n  dim: [" << dim_[0] << " " << dim_[1] << " " << dim_[2] << " "
@@ -1123,6 +1139,34 @@ private:
         makePair(sdom<7>().bdry<2>(), sdom<34>().bdry<2>(), transl, rot);
         makePair(sdom<8>().bdry<2>(), sdom<33>().bdry<2>(), transl, rot);
         for (auto& c : cells_) addSdom(c.get());
+    }
+
+    // WeightF (domain.cpp:1260-1280) over Field's columns: each cell's
+    // columns run with the first index fastest (field.cpp:25-45)
+    std::vector<double> weight_;
+    void weigh() {
+        const mcg_domain_spec& d = spec();
+        int64_t n = 0;
+        mcg::check(mcg_cell_volumes(&d, nullptr, 0, &n));
+        std::vector<double> vol(size_t(n), 0.0);
+        mcg::check(mcg_cell_volumes(&d, vol.data(), n, &n));
+        weight_.assign(size_t(n), 0.0);
+        int64_t col = 0;
+        for (int i = 0; i < d.nsdom; ++i) {
+            const mcg_subdomain_spec& c = d.sdom[i];
+            int64_t sh[3] = {1, 1, 1};
+            if (c.cell == MCG_PRISM || c.cell == MCG_PYRAMID) {
+                if (c.div[0] < 0) sh[0] = sh[1] = sh[2] = 0;
+            } else if (c.div[0] < 0 || c.div[1] < 0 || c.div[2] < 0) {
+                sh[0] = sh[1] = sh[2] = 0;
+            } else {
+                for (int k = 0; k < 3; ++k) sh[k] = std::max<int64_t>(c.div[k], 1);
+            }
+            const int64_t per_layer = sh[0] * sh[1], total = per_layer * sh[2];
+            if (i >= 16 && i < 26)
+                for (int64_t q = 0; q < per_layer; ++q) weight_[size_t(col + q)] = vol[size_t(col + q)];
+            col += total;
+        }
     }
 };
 
@@ -1369,7 +1413,14 @@ inline ArrayXXd solveField(const FieldProblem* prob, Rng& gen, std::ostream* log
         *log << "Output\n";
         printSolution(sol, *log);
     }
-    return prob->dom()->average(sol);
+    ArrayXXd avg = prob->dom()->average(sol);
+    bool same = avg.rows() == sol.rows() && avg.cols() == sol.cols();
+    for (long i = 0; same && i < sol.size(); ++i) same = avg.data()[i] == sol.data()[i];
+    if (log && !same) {
+        *log << "Averaged\n";
+        printSolution(avg, *log);
+    }
+    return avg;
 }
 
 // solveField(prob, nsim) + calcStats (main.cpp:181-210)

@@ -237,3 +237,32 @@ def checkpoints(dim, div=(0, 0, 0, 0)):
     """OctetDomain::checkpoints (domain.cpp:1247-1250): 52 points inside the
     truss, two to three per cell, as 52 x 3."""
     return cells(dim, div)[1]
+
+
+def average_weights(dom, cell_volumes):
+    """OctetDomain::WeightF (domain.cpp:1260-1280) over the field's columns:
+    a strut's (cells 16-25) cell volume on its first layer along its third
+    axis, zero elsewhere.  ``cell_volumes`` are the per-column volumes (for
+    example ``engine.cell_volumes(dom)``)."""
+    w = np.zeros(len(cell_volumes))
+    col = 0
+    for i, c in enumerate(dom.cells):
+        if c.cell in (abi.PRISM, abi.PYRAMID):
+            sh = (0, 0, 0) if c.div[0] < 0 else (1, 1, 1)
+        elif min(c.div[:3]) < 0:
+            sh = (0, 0, 0)
+        else:
+            sh = tuple(max(int(v), 1) for v in c.div[:3])
+        layer = sh[0] * sh[1]
+        if 16 <= i < 26:
+            w[col:col + layer] = cell_volumes[col:col + layer]
+        col += layer * sh[2]
+    return w
+
+
+def average(data, dom, cell_volumes):
+    """OctetDomain::average (domain.cpp:1252-1258): the rows x cols field
+    reduced to one column, weighted by :func:`average_weights`."""
+    w = average_weights(dom, cell_volumes)
+    return (np.asarray(data) * w).sum(axis=1, keepdims=True) / w.sum()
+

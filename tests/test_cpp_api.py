@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from montecarlocpp_amd import abi, build, materials, scenes
+from montecarlocpp_amd import abi, build, materials, octet, scenes
 
 DRIVER = build.DRIVER
 
@@ -102,9 +102,9 @@ def test_driver_errors(deckdir):
     assert r.returncode == 1 and "Error opening dispersion file" in r.stderr
 
 
-def parse_solution(out):
+def parse_solution(out, block="Output"):
     lines = out.splitlines()
-    k = lines.index("Output")
+    k = lines.index(block)
     rows = []
     for l in lines[k + 1:]:
         if not l.strip():
@@ -153,5 +153,11 @@ def test_driver_matches_checker(deckdir, deck):
         tally, nemit, size, maxscat = abi.CUMFLUX, deck[6], deck[7], deck[8]
     f, _, _ = oracle.solve(mat, dom, scenes.problem(nemit, tally=tally, maxscat=maxscat, size=size,
                                                       seed=key))
+    if kind == "octet":   # solveField also prints dom->average(sol) (main.cpp:171-177)
+        avg, _ = parse_solution(r.stdout, "Averaged")
+        want = octet.average(f, dom, oracle.cell_volumes(dom))
+        np.testing.assert_allclose(avg, want, rtol=1e-8, atol=1e-9 * np.abs(want).max())
+    else:
+        assert "Averaged" not in r.stdout
     assert sol.shape == f.shape
     np.testing.assert_allclose(sol, f, rtol=1e-8, atol=1e-9 * np.abs(f).max())

@@ -87,3 +87,16 @@ def test_invalid_dimensions():
         octet.octet((6e-7, 1e-7, 1e-7, 4e-8), DIV, 1.0)   # b <= t
     with pytest.raises(ValueError, match="Invalid dimensions"):
         octet.octet((3e-7, 1e-7, 1e-7, 1e-8), DIV, 1.0)   # period too short
+
+
+def test_average_weights():
+    """OctetDomain::average weighs the struts' first layer along their third
+    axis by cell volume: with div (4, 2, 2, 1) that is 16 cells; a constant
+    field averages to itself."""
+    d = octet.octet(DIM, DIV, 1.0)
+    vol = engine.cell_volumes(d)
+    w = octet.average_weights(d, vol)
+    assert np.count_nonzero(w) == 16 and np.all(w[w > 0] == vol[w > 0])
+    field = np.vstack([np.full(vol.size, 3.0), np.arange(vol.size, dtype=float)])
+    avg = octet.average(field, d, vol)
+    assert avg.shape == (2, 1) and avg[0, 0] == pytest.approx(3.0, rel=1e-14)
