@@ -2,12 +2,14 @@
 oracle_vectors.npz: one small solve per feature family (1-D slab with
 isothermal and specular walls, diffuse bar, periodic film with a volume
 source, Inter-joined tee, porous cell on a 2-D map, triangular prism, a
-cumulative tally kind).  The reference holds no golden vectors of its own,
+cumulative tally kind, a pentagonal prism with a volume source, the octet
+truss).  The reference holds no golden vectors of its own,
 so these record the FP64 oracle's per-bundle counts and tallies; they catch
 any change to the bundle streams or the arithmetic of either side."""
 import numpy as np
 
-from montecarlocpp_amd import abi, materials, scenes
+from montecarlocpp_amd import abi, materials, octet, scenes
+from tests import ngon_scenes
 
 NBUNDLES = 2000
 
@@ -28,4 +30,7 @@ def cases():
         "prism": (si, prism, scenes.problem(NBUNDLES, maxscat=60, seed=6)),
         "cumflux": (materials.si_like(200.0), scenes.slab(2e-7, 10), scenes.problem(NBUNDLES, tally=abi.CUMFLUX,
                                                                                       size=4, seed=7)),
+        "ngon": (si, ngon_scenes.prism(), scenes.problem(NBUNDLES, maxscat=60, seed=8)),
+        "octet": (si, octet.octet((6e-7, 1e-7, 1e-7, 1e-8), (4, 2, 2, 1), 1.0),
+                  scenes.problem(NBUNDLES, maxscat=40, seed=9)),
     }
