@@ -118,6 +118,23 @@ def test_bundle_subrange(si300):
     assert np.array_equal(cf.nstep[5000:12000], cp.nstep)
 
 
+def test_bundle_ids_past_32_bits(si300):
+    """Bundle ids past 2^32 (a 2^33-bundle solve, run over the range that
+    straddles 2^32, which is also where the second wall's bundles start): the
+    ids' high words reach the streams and the emitter search, bundle for
+    bundle as in the checker; FP32 flight runs the same range."""
+    dom = scenes.slab(100e-9, 20)
+    first = (1 << 32) - 3000
+    prob = scenes.problem(1 << 33, seed=21, first=first, count=6000)
+    r = run_both(si300, dom, prob)
+    check_det(*r)
+    assert r[1]["bundles"] == 6000
+    p32 = scenes.problem(1 << 33, seed=21, first=first, count=6000, precision=abi.FP32_FLIGHT)
+    with engine.Scene(si300, dom) as sc:
+        _, st, _ = sc.solve(p32)
+    assert st["bundles"] == 6000 and st["escapes"] <= 6
+
+
 @pytest.mark.parametrize("name", ["jct", "tee", "tube"])
 def test_assemblies_fp64(si300, name):
     """The reference's multi-box assemblies (domain.cpp:338-540): Inter faces
