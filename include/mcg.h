@@ -270,6 +270,20 @@ int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob,
 int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, double* rec,
               int64_t cap, int64_t* nrec);
 
+/* TrajProblem(mat, dom, [prop,] pos, [dir,] maxscat, maxloop)::solve with a
+ * given start (problem.cpp:226-299, the pos_ branch; constructors
+ * problem.cpp:163-198): one trajectory from pos.  prop_cell < 0 draws the
+ * mode from the scattering distribution (drawScatProp) as the reference
+ * does, otherwise it is the (w, p) cell w * np + p, which must have a
+ * positive energy pdf; dir NULL draws an isotropic direction (drawIso), a
+ * given one is normalised.  The start cell is the first subdomain that
+ * contains pos (Domain::locate, domain.cpp:58-66); "Position not inside
+ * domain" otherwise.  The draws use event 0 of bundle prob->first_bundle of
+ * prob's seed; maxscat, maxloop and precision come from prob.  Records and
+ * nrec as mcg_trace; the fate of the last record tells how it ended. */
+int mcg_trace_from(mcg_scene* scene, const mcg_problem_spec* prob, int64_t prop_cell,
+                   const double pos[3], const double* dir, double* rec, int64_t cap, int64_t* nrec);
+
 /* --------------------------------------------------------------- batching
  * Many independent solves in one persistent launch.  Each solve has its
  * own scene, problem and output; results equal those of mcg_solve. */

@@ -1164,6 +1164,41 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     }
 }
 
+// A trajectory from a given start (TrajProblem::solve with pos_ set,
+// problem.cpp:226-299): one lane, event 0 of bundle K.trace_id for the mode
+// (scattering distribution unless given), the direction (isotropic unless
+// given) and the first free path, then the solve's step loop.
+template <typename F>
+struct TraceStart {
+    double pos[3], dir[3];
+    int32_t has_dir, mode, sdom, pad;
+};
+
+template <typename F>
+__global__ void __launch_bounds__(64) trace_from_kernel(KParams<F> K, TraceStart<F> s) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const View<F> V = stage<F, false, false>(K, lds, threadIdx.x, blockDim.x);
+    const Sink sink{V.direct, V.diff, K.cols};
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const dev::Event ev{K.k0, K.k1, 0u, uint32_t(K.trace_id), uint32_t(uint64_t(K.trace_id) >> 32)};
+    const uint4 b0 = ev.block(0), b1 = low_block<F>(ev, 1);
+    Lane<F> L;
+    L.mode = s.mode >= 0 ? uint32_t(s.mode) : dev::alias_draw(K.scat_alias, uint32_t(K.nmodes), b0.w);
+    const dev::V3<F> d = s.has_dir ? dev::V3<F>{F(s.dir[0]), F(s.dir[1]), F(s.dir[2])} : dev::draw_iso<F>(b0, b1);
+    L.dir = dev::normalized(d);
+    L.pos = dev::V3<F>{F(s.pos[0]), F(s.pos[1]), F(s.pos[2])};
+    L.sdom = s.sdom;
+    L.sign = 1;
+    L.scat_next = draw_path<F>(ev, b0, b1, V.mfp[L.mode]);
+    L.nscat = L.nbdry = L.nstep = 0;
+    L.id = K.trace_id;
+    Walk<F> W;
+    bool walking = false;
+    while (scatter_step<F, false>(K, V, sink, L, W, walking) == 0xFFFFFFFFu) {
+    }
+}
+
 // Many solves in one persistent launch.  Workgroups start spread over the
 // solves and then visit every solve once, in order: a solve with bundles
 // left is staged in LDS (tables, faces, tally) and its bundles are claimed
@@ -1612,6 +1647,52 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     }
     if (int64_t(hc[C_BUNDLES]) != p.count)
         return set_error(MCG_EDEVICE, "kernel finished with bundles unaccounted for");
+    return MCG_OK;
+}
+
+// Host side of mcg_trace_from: buffers for the one lane's tallies (kept on
+// the GPU and dropped), its counts and its records.
+template <typename F>
+int run_trace_from(mcg_scene* sc, const HostProblem& p, const TraceStart<F>& start, double* trace,
+                   int64_t cap, int64_t* nrec) {
+    const HostScene& h = sc->host;
+    HIP_TRY(hipSetDevice(sc->device));
+    const int rows = int(p.rows), cols = int(h.cols);
+    if (p.rows > (1 << 20) || h.cols * p.rows > (1ll << 31)) return set_error(MCG_ERANGE, "tally too large");
+    const size_t nd = size_t(rows) * cols, nf = size_t(rows) * (cols + 1);
+    DevBuf<double> tally, tbuf;
+    DevBuf<uint32_t> cnt;
+    int rc;
+    if ((rc = tally.alloc(nd + nf + 1))) return rc;
+    if ((rc = cnt.alloc(4))) return rc;
+    if ((rc = tbuf.alloc(size_t(cap) * MCG_TRACE_WIDTH))) return rc;
+    HIP_TRY(hipMemset(tally.p, 0, tally.n * sizeof(double)));
+    HIP_TRY(hipMemset(cnt.p, 0, cnt.n * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(tbuf.p, 0, tbuf.n * sizeof(double)));
+    KParams<F> K{};
+    fill_params<F>(sc, p, K);
+    K.direct = tally.p;
+    K.diff = tally.p + nd;
+    K.c_nscat = cnt.p;
+    K.c_nbdry = cnt.p + 1;
+    K.c_nstep = cnt.p + 2;
+    K.c_fate = cnt.p + 3;
+    K.trace_id = p.first;
+    K.trace = tbuf.p;
+    K.trace_cap = cap;
+    const size_t lds = lds_bytes<F>(h, rows, false, false);
+    if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
+    const void* fn = reinterpret_cast<const void*>(trace_from_kernel<F>);
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    TraceStart<F> s = start;
+    void* args[] = {&K, &s};
+    HIP_TRY(hipLaunchKernel(fn, dim3(1), dim3(kWave), args, lds, 0));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    uint32_t hc[4];
+    HIP_TRY(hipMemcpy(hc, cnt.p, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(trace, tbuf.p, tbuf.n * sizeof(double), hipMemcpyDeviceToHost));
+    *nrec = std::min<int64_t>(int64_t(hc[2]), cap);
     return MCG_OK;
 }
 
@@ -2327,6 +2408,59 @@ int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, do
     if (rc) return rc;
     *nrec = std::min<int64_t>(int64_t(c_nstep), cap);
     return MCG_OK;
+}
+
+int mcg_trace_from(mcg_scene* scene, const mcg_problem_spec* prob, int64_t prop_cell,
+                   const double pos[3], const double* dir, double* rec, int64_t cap, int64_t* nrec) {
+    if (!scene || !prob || !pos || !rec || !nrec || cap <= 0) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    const mcg::HostScene& h = scene->host;
+    if (prob->first_bundle < 0) return mcg::set_error(MCG_ERANGE, "bundle id must be non-negative");
+    mcg_problem_spec one = *prob;
+    one.first_bundle = 0;
+    one.bundle_count = 1;
+    mcg::HostProblem p;
+    int rc = mcg::build_problem(&one, h, p);
+    if (rc) return rc;
+    p.first = prob->first_bundle;   // the stream's id; it need not be an emitted bundle
+    int mode = -1;
+    if (prop_cell >= 0) {
+        for (size_t k = 0; k < h.mode_cell.size(); ++k)
+            if (h.mode_cell[k] == prop_cell) mode = int(k);
+        if (mode < 0) return mcg::set_error(MCG_EINVAL, "prop has no energy pdf");
+    }
+    // Domain::locate: the first subdomain whose faces all hold pos within eps
+    int sdom = -1;
+    for (size_t si = 0; si < h.sdoms.size() && sdom < 0; ++si) {
+        const mcg::DevSdom<double>& S = h.sdoms[si];
+        bool in = true;
+        for (int b = S.pbeg; b < S.pend && in; ++b) {
+            const mcg::DevPlane<double>& P = h.planes[b];
+            in = P.n[0] * pos[0] + P.n[1] * pos[1] + P.n[2] * pos[2] + P.c >= -S.eps;
+        }
+        if (in && S.pore >= 0) {
+            const mcg::DevPore<double>& Q = h.pores[S.pore];
+            const double dx = pos[0] - Q.cx, dy = pos[1] - Q.cy, rr = Q.r - Q.eps;
+            in = !(dx * dx + dy * dy < rr * rr);
+        }
+        if (in) sdom = int(si);
+    }
+    if (sdom < 0) return mcg::set_error(MCG_EINVAL, "Position not inside domain");
+    if (dir && !(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2] > 0.0))
+        return mcg::set_error(MCG_EINVAL, "direction must be nonzero");
+    if (p.precision == MCG_FP64) {
+        mcg::TraceStart<double> s{};
+        for (int k = 0; k < 3; ++k) { s.pos[k] = pos[k]; s.dir[k] = dir ? dir[k] : 0.0; }
+        s.has_dir = dir ? 1 : 0;
+        s.mode = mode;
+        s.sdom = sdom;
+        return mcg::run_trace_from<double>(scene, p, s, rec, cap, nrec);
+    }
+    mcg::TraceStart<float> s{};
+    for (int k = 0; k < 3; ++k) { s.pos[k] = pos[k]; s.dir[k] = dir ? dir[k] : 0.0; }
+    s.has_dir = dir ? 1 : 0;
+    s.mode = mode;
+    s.sdom = sdom;
+    return mcg::run_trace_from<float>(scene, p, s, rec, cap, nrec);
 }
 
 int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,

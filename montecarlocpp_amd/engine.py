@@ -21,7 +21,7 @@ EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_pro
            "mcg_server_create", "mcg_server_submit", "mcg_server_step", "mcg_server_completed",
            "mcg_server_poll",
            "mcg_server_cancel", "mcg_server_info", "mcg_server_destroy",
-           "mcg_solve", "mcg_trace", "mcg_batch_solve", "mcg_batch_create", "mcg_batch_run",
+           "mcg_solve", "mcg_trace", "mcg_trace_from", "mcg_batch_solve", "mcg_batch_create", "mcg_batch_run",
            "mcg_batch_destroy")
 
 TRACE_WIDTH = 16
@@ -78,6 +78,9 @@ def lib():
                                 C.POINTER(abi.Counts), C.POINTER(abi.Stats)]
         L.mcg_trace.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double), C.c_int64,
                                 C.POINTER(C.c_int64)]
+        L.mcg_trace_from.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                                     C.POINTER(C.c_int64)]
         L.mcg_batch_solve.argtypes = [C.c_int32, C.POINTER(C.c_void_p), PS,
                                       C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
                                       C.POINTER(abi.Stats), C.POINTER(abi.Tune)]
@@ -204,6 +207,17 @@ class Scene:
         n = C.c_int64()
         check(lib().mcg_trace(self.handle, C.byref(prob), int(bundle), abi.dptr(rec), cap,
                               C.byref(n)))
+        return rec.reshape(cap, TRACE_WIDTH)[:n.value].copy()
+
+    def trace_from(self, prob, pos, dir=None, prop_cell=-1, cap=4096):
+        """Step records of one trajectory from a given start (mcg_trace_from,
+        TrajProblem with pos): array (nsteps, 16)."""
+        rec = np.zeros(cap * TRACE_WIDTH)
+        n = C.c_int64()
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        d = None if dir is None else np.ascontiguousarray(dir, dtype=np.float64)
+        check(lib().mcg_trace_from(self.handle, C.byref(prob), int(prop_cell), abi.dptr(p),
+                                   None if d is None else abi.dptr(d), abi.dptr(rec), cap, C.byref(n)))
         return rec.reshape(cap, TRACE_WIDTH)[:n.value].copy()
 
 

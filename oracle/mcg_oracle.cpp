@@ -1368,6 +1368,11 @@ void record(double* trace, int64_t cap, uint32_t nstep, int hit, double dist, V3
     r[15] = ph.pos.y;
 }
 
+// The step loop of FieldProblem::solve (problem.cpp:401-437) for bundle n
+// starting as ph in subdomain sdom.
+void run_phonon(const Material& M, const Domain& D, const Problem& P, int64_t n, Phonon ph, int sdom,
+                Field& F, Counters& C, mcg_counts* counts, double* trace, int64_t cap);
+
 void run_bundle(const Material& M, const Domain& D, const Problem& P, int64_t n, Field& F,
                 Counters& C, mcg_counts* counts, double* trace = nullptr, int64_t cap = 0) {
     int64_t emitIndex = std::upper_bound(P.emitCdf.begin(), P.emitCdf.end(), n) - P.emitCdf.begin();
@@ -1375,6 +1380,11 @@ void run_bundle(const Material& M, const Domain& D, const Problem& P, int64_t n,
     EventRng g0(P.seed, 0u, uint64_t(n));
     int sdom;
     Phonon ph = emit(M, D, e, g0, sdom);
+    run_phonon(M, D, P, n, ph, sdom, F, C, counts, trace, cap);
+}
+
+void run_phonon(const Material& M, const Domain& D, const Problem& P, int64_t n, Phonon ph, int sdom,
+                Field& F, Counters& C, mcg_counts* counts, double* trace, int64_t cap) {
     uint32_t nbdry = 0, nstep = 0, fate = MCG_MAXLOOP;
     double amt[64];
     std::vector<double> amtv;
@@ -1736,6 +1746,56 @@ int oracle_trace(const mcg_material_spec* mat, const mcg_domain_spec* dom,
     Problem P1 = B.P;
     P1.first = bundle;
     run_bundle(B.M, B.D, P1, bundle, F, C, &counts, rec, cap);
+    *nrec = std::min<int64_t>(nt, cap);
+    return MCG_OK;
+}
+
+// TrajProblem::solve from a given start (problem.cpp:226-299, pos_ set):
+// the mode from the scattering distribution unless given (drawScatProp),
+// an isotropic direction unless given (drawIso), the first free path, all
+// from event 0 of bundle prob->first_bundle; the start cell by
+// Domain::locate (domain.cpp:58-66).
+int oracle_trace_from(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                      const mcg_problem_spec* prob, int64_t prop_cell, const double pos[3],
+                      const double* dir, double* rec, int64_t cap, int64_t* nrec) {
+    if (!prob || !pos || !rec || !nrec || cap <= 0) return fail(MCG_EINVAL, "NULL argument");
+    if (prob->first_bundle < 0) return fail(MCG_ERANGE, "bundle id must be non-negative");
+    mcg_problem_spec one = *prob;
+    one.first_bundle = 0;
+    one.bundle_count = 1;
+    Built B;
+    int rc = build_all(mat, dom, &one, B);
+    if (rc) return rc;
+    B.P.first = prob->first_bundle;   // the stream's id; it need not be an emitted bundle
+    int mode = -1;
+    if (prop_cell >= 0) {
+        for (size_t k = 0; k < B.M.modeCell.size(); ++k)
+            if (B.M.modeCell[k] == prop_cell) mode = int(k);
+        if (mode < 0) return fail(MCG_EINVAL, "prop has no energy pdf");
+    }
+    const V3 p = v3(pos[0], pos[1], pos[2]);
+    int sdom = -1;
+    for (size_t si = 0; si < B.D.sdoms.size() && sdom < 0; ++si)
+        if (is_inside(B.D, B.D.sdoms[si], p)) sdom = int(si);
+    if (sdom < 0) return fail(MCG_EINVAL, "Position not inside domain");
+    if (dir && !(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2] > 0.0))
+        return fail(MCG_EINVAL, "direction must be nonzero");
+    const int64_t n = B.P.first;
+    EventRng g(B.P.seed, 0u, uint64_t(n));
+    Phonon ph{};
+    ph.mode = mode >= 0 ? mode : alias_draw(B.M.scatAlias, g.word(W_MODE));
+    ph.dir = normalized(dir ? v3(dir[0], dir[1], dir[2]) : draw_iso(g));
+    ph.pos = p;
+    ph.alive = true;
+    ph.sign = 1;
+    ph.scatNext = 0.0;
+    ph.nscat = 0;
+    draw_scat_next(B.M, ph, g);
+    Field F{B.P.rows, B.D.cols, std::vector<double>(B.P.rows * B.D.cols, 0.0)};
+    Counters C;
+    uint32_t ns, nb, nt, fa;
+    mcg_counts counts{&ns, &nb, &nt, &fa};
+    run_phonon(B.M, B.D, B.P, n, ph, sdom, F, C, &counts, rec, cap);
     *nrec = std::min<int64_t>(nt, cap);
     return MCG_OK;
 }

@@ -56,6 +56,10 @@ def lib():
         L.oracle_trace.argtypes = [C.POINTER(abi.MaterialSpec), C.POINTER(abi.DomainSpec),
                                    C.POINTER(abi.ProblemSpec), C.c_int64, C.POINTER(C.c_double),
                                    C.c_int64, C.POINTER(C.c_int64)]
+        L.oracle_trace_from.argtypes = [C.POINTER(abi.MaterialSpec), C.POINTER(abi.DomainSpec),
+                                        C.POINTER(abi.ProblemSpec), C.c_int64, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                                        C.POINTER(C.c_int64)]
         L.oracle_math.argtypes = [C.c_int32, C.POINTER(C.c_double), C.c_int64,
                                   C.POINTER(C.c_double)]
         L.oracle_math.restype = None
@@ -179,3 +183,16 @@ def disk_area(q):
     out = np.zeros(len(q))
     lib().oracle_disk_area(abi.dptr(q), len(q), abi.dptr(out))
     return out
+
+
+def trace_from(mat, dom, prob, pos, dir=None, prop_cell=-1, cap=4096):
+    """TrajProblem with a given start: the checker's records, as engine.Scene.trace_from."""
+    rec = np.zeros(cap * 16)
+    n = C.c_int64()
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    d = None if dir is None else np.ascontiguousarray(dir, dtype=np.float64)
+    _check(lib().oracle_trace_from(C.byref(mat.spec), C.byref(dom.spec()), C.byref(prob), int(prop_cell),
+                                   abi.dptr(p), None if d is None else abi.dptr(d), abi.dptr(rec), cap,
+                                   C.byref(n)))
+    return rec.reshape(cap, 16)[:n.value].copy()
+
