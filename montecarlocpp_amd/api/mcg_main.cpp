@@ -164,6 +164,27 @@ int main(int argc, const char* argv[]) {
 
         argss >> probStr;
         long nemit = 0, size = 0, maxscat = 0, maxloop = 0, nsim = 0;
+        if (probStr == "traj") {
+            // main.cpp:416-426: pos, dir, maxscat, maxloop; solveTraj (main.cpp:86-102)
+            Vector3d pos, dir;
+            argss >> pos(0) >> pos(1) >> pos(2) >> dir(0) >> dir(1) >> dir(2) >> maxscat >> maxloop;
+            if (argss.fail()) throw mcg::Error(MCG_EINVAL, "Invalid problem arguments");
+            TrajProblem tp(mat.get(), dom.get(), pos, dir, maxscat, maxloop);
+            const char* pr = std::getenv("MCG_PRECISION");
+            tp.precision(pr && std::string(pr) == "fp32" ? MCG_FP32_FLIGHT : MCG_FP64);
+            std::cout << tp << "\n\n";
+            const char* se = std::getenv("MCG_SEED");
+            const uint64_t seed = se ? std::strtoull(se, nullptr, 10) : uint64_t(std::random_device()());
+            Rng gen(seed);
+            std::cout << "Trajectory 0\n  seeds: " << seed << "\n";
+            ArrayXXd traj = tp.solve(gen, nullptr);
+            const std::vector<double>& h = tp.hops();
+            const size_t nh = h.size() / MCG_TRACE_WIDTH;
+            if (nh > 0 && h[(nh - 1) * MCG_TRACE_WIDTH + 13] == double(MCG_ESCAPED)) std::cout << "Escaped\n";
+            printSolution(traj);
+            std::cout << "  key: " << tp.lastKey() << "\n";
+            return 0;
+        }
         std::unique_ptr<FieldProblem> prob;
         if (probStr == "temp" || probStr == "flux" || probStr == "multi") {
             argss >> nemit >> maxscat >> maxloop >> nsim;

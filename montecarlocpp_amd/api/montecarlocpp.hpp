@@ -1303,6 +1303,75 @@ public:
     virtual ArrayXXd solve(Rng& gen, Progress* prog) const = 0;
 };
 
+// TrajProblem(mat, dom, [prop,] pos, [dir,] maxscat, maxloop) with a given
+// start (problem.h:88-119, problem.cpp:163-299): one trajectory on the GPU
+// (mcg_trace_from).  solve() returns the points the bundle visits, 3 x n: the
+// start, then the end of every flight; hops() holds the step records.
+class TrajProblem : public Problem {
+    mutable mcg_scene* scene_ = nullptr;
+    long maxscat_, maxloop_;
+    long prop_ = -1;                  // (w, p) cell w * np + p, or -1: drawn
+    Vector3d pos_, dir_;
+    bool hasDir_;
+    int precision_ = MCG_FP64;
+    mutable std::vector<double> rec_;
+    mutable uint64_t lastKey_ = 0;
+
+public:
+    TrajProblem(const Material* mat, Domain* dom, const Vector3d& pos, const Vector3d& dir,
+                long maxscat = 100, long maxloop = 0)
+        : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), pos_(pos), dir_(dir), hasDir_(true) {}
+    TrajProblem(const Material* mat, Domain* dom, const Vector3d& pos, long maxscat = 100, long maxloop = 0)
+        : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), pos_(pos), hasDir_(false) {}
+    TrajProblem(const Material* mat, Domain* dom, long propCell, const Vector3d& pos, const Vector3d& dir,
+                long maxscat = 100, long maxloop = 0)
+        : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), prop_(propCell), pos_(pos), dir_(dir),
+          hasDir_(true) {}
+    TrajProblem(const TrajProblem&) = delete;
+    ~TrajProblem() override {
+        if (scene_) mcg_scene_destroy(scene_);
+    }
+    void precision(int p) { precision_ = p; }
+    uint64_t lastKey() const { return lastKey_; }
+    Progress initProgress() const override { return Progress(); }
+    ArrayXXd solve(Rng& gen, Progress*) const override {
+        if (!scene_) {
+            mcg_material_spec ms = mat()->spec();
+            mcg::check(mcg_scene_create(&ms, &dom()->spec(), &scene_));
+        }
+        mcg_problem_spec ps{};
+        ps.tally = MCG_TEMP;
+        ps.precision = precision_;
+        ps.nemit = 1;
+        ps.maxscat = maxscat_;
+        ps.maxloop = maxloop_;
+        lastKey_ = gen();
+        ps.seed = lastKey_;
+        const int64_t cap = std::max<long>(1, maxloop_ != 0 ? maxloop_ : 100 * maxscat_);
+        rec_.assign(size_t(std::min<int64_t>(cap, 1 << 20)) * MCG_TRACE_WIDTH, 0.0);
+        int64_t n = 0;
+        const double p[3] = {pos_(0), pos_(1), pos_(2)}, d[3] = {dir_(0), dir_(1), dir_(2)};
+        mcg::check(mcg_trace_from(scene_, &ps, prop_, p, hasDir_ ? d : nullptr, rec_.data(),
+                                  int64_t(rec_.size() / MCG_TRACE_WIDTH), &n));
+        rec_.resize(size_t(n) * MCG_TRACE_WIDTH);
+        ArrayXXd traj(3, n + 1);
+        for (int k = 0; k < 3; ++k) traj(k, 0) = pos_(k);
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) traj(k, i + 1) = rec_[size_t(i) * MCG_TRACE_WIDTH + 3 + k];
+        return traj;
+    }
+    // the step records of the last solve (MCG_TRACE_WIDTH doubles each)
+    const std::vector<double>& hops() const { return rec_; }
+    std::string info() const {
+        std::ostringstream ss;
+        ss << "TrajProblem " << this << "\n  pos:     [" << pos_(0) << " " << pos_(1) << " " << pos_(2) << "]";
+        if (hasDir_) ss << "\n  dir:     [" << dir_(0) << " " << dir_(1) << " " << dir_(2) << "]";
+        ss << "\n  maxscat: " << maxscat_ << "\n  maxloop: " << maxloop_;
+        return ss.str();
+    }
+    friend std::ostream& operator<<(std::ostream& os, const TrajProblem& p) { return os << p.info(); }
+};
+
 // FieldProblem(mat, dom, nemit, maxscat, maxloop) (problem.cpp:315-342).
 // The engine rounds the per-emitter counts and computes power_; solve() runs
 // one GPU solve and returns power_ * postProc(data) / cellVol.

@@ -161,3 +161,30 @@ def test_driver_matches_checker(deckdir, deck):
         assert "Averaged" not in r.stdout
     assert sol.shape == f.shape
     np.testing.assert_allclose(sol, f, rtol=1e-8, atol=1e-9 * np.abs(f).max())
+
+
+@pytest.mark.gpu
+def test_driver_traj_matches_checker(deckdir):
+    """The traj mode (main.cpp:416-426): one trajectory from a given start,
+    printed as the points it visits, equal to the checker's from the same
+    key."""
+    from oracle import oracle
+    deck = ["silicon", 300, "film", 2e-7, 5e-8, 6, "traj", 5e-8, 2e-8, 1e-7, 0.6, 0.0, 0.8, 20, 0]
+    print("deck:", deck, flush=True)
+    r = run([deckdir] + deck, env={"MCG_SEED": "11"})
+    lines = r.stdout.splitlines()
+    k = lines.index("Trajectory 0")
+    rows = []
+    for l in lines[k + 2:]:
+        if not l.strip() or l.strip().startswith("key"):
+            break
+        rows.append([float(t) for t in l.split()])
+    traj = np.array(rows)
+    key = int(re.search(r"key: (\d+)", r.stdout).group(1))
+    mat = materials.read_reference_files(deckdir / "Si_disp.txt", deckdir / "Si_relax2.txt", 300.0)
+    dom = scenes.film((2e-7, 5e-8, 2e-7), (0, 6, 0), 1e6 * 2e-7)
+    rec = oracle.trace_from(mat, dom, scenes.problem(1, maxscat=20, seed=key), (5e-8, 2e-8, 1e-7), (0.6, 0.0, 0.8))
+    want = np.vstack([[5e-8, 2e-8, 1e-7], rec[:, 3:6]]).T
+    assert traj.shape == want.shape
+    np.testing.assert_allclose(traj, want, rtol=1e-8, atol=1e-16)
+
