@@ -279,8 +279,10 @@ int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, do
  * given one is normalised.  The start cell is the first subdomain that
  * contains pos (Domain::locate, domain.cpp:58-66); "Position not inside
  * domain" otherwise.  The draws use event 0 of bundle prob->first_bundle of
- * prob's seed; maxscat, maxloop and precision come from prob.  Records and
- * nrec as mcg_trace; the fate of the last record tells how it ended. */
+ * prob's seed; maxscat, maxloop and precision come from prob, which must be
+ * a valid problem of the scene (so the domain needs a source, as for a
+ * solve).  Records and nrec as mcg_trace; the fate of the last record tells
+ * how it ended. */
 int mcg_trace_from(mcg_scene* scene, const mcg_problem_spec* prob, int64_t prop_cell,
                    const double pos[3], const double* dir, double* rec, int64_t cap, int64_t* nrec);
 
