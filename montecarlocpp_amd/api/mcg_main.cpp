@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <ctime>
 #include <iomanip>
+#include <limits>
 
 #include "montecarlocpp.hpp"
 
@@ -164,6 +165,48 @@ int main(int argc, const char* argv[]) {
 
         argss >> probStr;
         long nemit = 0, size = 0, maxscat = 0, maxloop = 0, nsim = 0;
+        if (probStr == "check") {
+            // main.cpp:407-414 and checkDomain (main.cpp:104-143): from every
+            // check point, a two-scatter trajectory along each column of rot
+            // and its opposite; then all of them, NaN-separated
+            Matrix3d rot;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) argss >> rot(r, c);
+            if (argss.fail()) throw mcg::Error(MCG_EINVAL, "Invalid problem arguments");
+            const Matrix3Xd pts = dom->checkpoints();
+            const char* se = std::getenv("MCG_SEED");
+            const uint64_t seed = se ? std::strtoull(se, nullptr, 10) : uint64_t(std::random_device()());
+            Rng gen(seed);
+            std::vector<ArrayXXd> trajs;
+            long total = 0, n = 0;
+            for (long q = 0; q < pts.cols(); ++q) {
+                const Vector3d pos = pts.col(int(q));
+                for (int i = 0; i < 6; ++i) {
+                    Vector3d dir = rot.col(i % 3);
+                    if (i >= 3) dir = dir * -1.0;
+                    TrajProblem tp(mat.get(), dom.get(), pos, dir, 2, 2);
+                    std::cout << tp << "\n\n";
+                    std::cout << "Trajectory " << n++ << "\n  seeds: " << seed << "\n";
+                    trajs.push_back(tp.solve(gen, nullptr));
+                    const std::vector<double>& h = tp.hops();
+                    const size_t nh = h.size() / MCG_TRACE_WIDTH;
+                    if (nh > 0 && h[(nh - 1) * MCG_TRACE_WIDTH + 13] == double(MCG_ESCAPED)) std::cout << "Escaped\n";
+                    printSolution(trajs.back());
+                    total += trajs.back().cols() + 1;
+                }
+            }
+            ArrayXXd all(3, std::max<long>(0, total - 1));
+            for (long i = 0; i < all.size(); ++i) all.data()[i] = std::numeric_limits<double>::quiet_NaN();
+            long j = 0;
+            for (const ArrayXXd& t : trajs) {
+                for (long c = 0; c < t.cols(); ++c)
+                    for (int r = 0; r < 3; ++r) all(r, j + c) = t(r, c);
+                j += t.cols() + 1;
+            }
+            std::cout << "Combined Trajectory\n";
+            printSolution(all);
+            return 0;
+        }
         if (probStr == "traj") {
             // main.cpp:416-426: pos, dir, maxscat, maxloop; solveTraj (main.cpp:86-102)
             Vector3d pos, dir;

@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
+#include <initializer_list>
 #include <iostream>
 #include <map>
 #include <memory>
@@ -697,6 +698,9 @@ public:
         return spec_;
     }
     virtual ArrayXXd average(const ArrayXXd& data) const { return data; }
+    // points inside the domain that the check mode probes from (the
+    // reference's checkpoints(), domain.h:68)
+    virtual Matrix3Xd checkpoints() const { throw mcg::Error(MCG_EINVAL, "domain has no check points"); }
     friend std::ostream& operator<<(std::ostream& os, const Domain& d) { return os << d.info(); }
 };
 
@@ -705,6 +709,17 @@ using Spec = SpecBoundary;
 using Diff = DiffBoundary;
 using Inter = InterBoundary;
 using IsotP = IsotBoundary<Parallelogram>;
+
+// a 3 x n point list from n rows of (x, y, z)
+inline Matrix3Xd points3(std::initializer_list<Vector3d> rows) {
+    Matrix3Xd m(3, long(rows.size()));
+    int k = 0;
+    for (const Vector3d& v : rows) {
+        for (int r = 0; r < 3; ++r) m(r, k) = v(r);
+        ++k;
+    }
+    return m;
+}
 
 // BulkDomain (domain.h:93-114, domain.cpp:137-148)
 class BulkDomain : public Domain {
@@ -724,6 +739,7 @@ public:
         makePair(sdom_.bdry<0>(), sdom_.bdry<3>(), Vector3d(dim(0), 0., 0.));
         addSdom(&sdom_);
     }
+    Matrix3Xd checkpoints() const override { return points3({dim_ * 0.5}); }   // domain.cpp:168-171
     std::string info() const override {
         std::ostringstream ss;
         ss << "BulkDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2)
@@ -750,6 +766,7 @@ public:
         makePair(sdom_.bdry<0>(), sdom_.bdry<3>(), Vector3d(dim(0), 0., 0.));
         addSdom(&sdom_);
     }
+    Matrix3Xd checkpoints() const override { return points3({dim_ * 0.5}); }   // domain.cpp:219-222
     std::string info() const override {
         std::ostringstream ss;
         ss << "FilmDomain " << this << "\n  dim: [" << dim_(0) << " " << dim_(1) << " " << dim_(2)
@@ -786,6 +803,11 @@ public:
         addSdom(&s0_);
         addSdom(&s1_);
         addSdom(&s2_);
+    }
+    Matrix3Xd checkpoints() const override {   // domain.cpp:404-413
+        const auto& d = dim_;
+        return points3({Vector3d(d[0], 0.5 * d[1], 0.5 * d[3]), Vector3d(0.5 * d[0], d[1] + 0.5 * d[2], 0.5 * d[3]),
+                        Vector3d(1.5 * d[0], d[1] + 0.5 * d[2], 0.5 * d[3])});
     }
     std::string info() const override {
         std::ostringstream ss;
@@ -825,6 +847,12 @@ public:
         addSdom(&s2_);
         addSdom(&s3_);
     }
+    Matrix3Xd checkpoints() const override {   // domain.cpp:483-493
+        const auto& d = dim_;
+        return points3({Vector3d(0.5 * d[0], 0.5 * d[2], 0.5 * d[4]), Vector3d(d[0] + 0.5 * d[1], 0.5 * d[2], 0.5 * d[4]),
+                        Vector3d(d[0] + 0.5 * d[1], d[2] + 0.5 * d[3], 0.5 * d[4]),
+                        Vector3d(1.5 * d[0] + d[1], 0.5 * d[2], 0.5 * d[4])});
+    }
     std::string info() const override {
         std::ostringstream ss;
         ss << "TeeDomain " << this << "\n  dim: [" << dim_[0] << " " << dim_[1] << " " << dim_[2] << " "
@@ -860,6 +888,12 @@ public:
         addSdom(&s0_);
         addSdom(&s1_);
         addSdom(&s2_);
+    }
+    Matrix3Xd checkpoints() const override {   // domain.cpp:561-570
+        const auto& d = dim_;
+        return points3({Vector3d(0.5 * d[0], d[1] + 0.5 * d[3], 0.5 * d[2]),
+                        Vector3d(0.5 * d[0], d[1] + 0.5 * d[3], d[2] + 0.5 * d[3]),
+                        Vector3d(0.5 * d[0], 0.5 * d[1], d[2] + 0.5 * d[3])});
     }
     std::string info() const override {
         std::ostringstream ss;
@@ -1080,8 +1114,10 @@ public:
         }
         init();
         weigh();
+        points(a, b, t);
     }
     OctetDomain(const OctetDomain&) = delete;
+    Matrix3Xd checkpoints() const override { return pts_; }
 
     // OctetDomain::average (domain.cpp:1252-1280): one column, the mean of the
     // tallies over the struts' (cells 16-25) first layer along their third
@@ -1139,6 +1175,61 @@ private:
         makePair(sdom<7>().bdry<2>(), sdom<34>().bdry<2>(), transl, rot);
         makePair(sdom<8>().bdry<2>(), sdom<33>().bdry<2>(), transl, rot);
         for (auto& c : cells_) addSdom(c.get());
+    }
+
+    // the 52 check points of domain.cpp:693-1221, from each cell's origin o
+    // and edge columns c: mostly o + (c0 + c2) / 2, with the reference's
+    // other rules for the cells named below
+    Matrix3Xd pts_;
+    void points(double a, double b, double t) {
+        const double r2 = std::sqrt(2.0), h = t / r2;
+        std::vector<Vector3d> p;
+        for (int i = 0; i < 42; ++i) {
+            const mcg_subdomain_spec c = cells_[size_t(i)]->spec();
+            const Vector3d o(c.origin[0], c.origin[1], c.origin[2]);
+            auto col = [&](int k) { return Vector3d(c.mat[3 * k], c.mat[3 * k + 1], c.mat[3 * k + 2]); };
+            const Vector3d c0 = col(0), c1 = col(1), c2 = col(2);
+            switch (i) {
+                case 5: case 36: {
+                    const Vector3d u(i == 5 ? t : -t, 0., 0.);
+                    p.push_back(o + c0 * 0.5 + (c2 + u) * (1. / 3.));
+                    p.push_back(o + (c0 + c1 + c2 + u) * 0.5);
+                    break;
+                }
+                case 8: case 33: {
+                    const Vector3d v(0., 0., i == 8 ? -a : a);
+                    p.push_back(o + (c0 + c1 + v) * 0.5);
+                    p.push_back(o + (c0 + c2 + v) * 0.5);
+                    break;
+                }
+                case 9: case 32: {
+                    const Vector3d u(i == 9 ? r2 * b : -r2 * b, 0., 0.), v(0., 0., i == 9 ? -b + t : b - t);
+                    p.push_back(o + (c0 + c2 + u) * 0.5);
+                    p.push_back(o + (c0 + c2 + v) * 0.5);
+                    p.push_back(o + (c0 + u + v) * 0.5);
+                    break;
+                }
+                case 12: case 14: case 27: case 29:
+                    p.push_back(o + (c0 + c1) * (1. / 3.) + c2 * 0.5);
+                    break;
+                case 13: case 28:
+                    p.push_back(o + (c0 + c1 + c2 + col(3)) * 0.2);
+                    break;
+                case 18: case 23: {
+                    const double e = (r2 + 1.) / 2. * b + h, sg = i == 18 ? 1. : -1.;
+                    const Vector3d u(sg * e, 0., sg * e);
+                    p.push_back(o + (c1 + c2 + u) * 0.5);
+                    p.push_back(o + (c0 + c1 + c2 + u) * 0.5);
+                    break;
+                }
+                default:
+                    if (i >= 16 && i < 26) p.push_back(o + (c0 + c1 + c2) * 0.5);
+                    else p.push_back(o + (c0 + c2) * 0.5);
+            }
+        }
+        pts_ = Matrix3Xd(3, long(p.size()));
+        for (size_t k = 0; k < p.size(); ++k)
+            for (int r = 0; r < 3; ++r) pts_(r, int(k)) = p[k](r);
     }
 
     // WeightF (domain.cpp:1260-1280) over Field's columns: each cell's

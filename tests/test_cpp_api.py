@@ -188,3 +188,23 @@ def test_driver_traj_matches_checker(deckdir):
     assert traj.shape == want.shape
     np.testing.assert_allclose(traj, want, rtol=1e-8, atol=1e-16)
 
+
+@pytest.mark.gpu
+def test_driver_check_mode_octet(deckdir):
+    """The check mode (main.cpp:407-414, checkDomain main.cpp:104-143) over the
+    octet truss: six two-scatter probes from each of the 52 check points of
+    domain.cpp:693-1221, none escaping; each probe starts at its check point."""
+    deck = ["silicon", 300, "octet", 6e-7, 1e-7, 1e-7, 1e-8, 4, 2, 2, 1, 1.0, "check", 1, 0, 0, 0, 1, 0, 0, 0, 1]
+    print("deck:", deck, flush=True)
+    r = run([deckdir] + deck, env={"MCG_SEED": "5"})
+    assert "Escaped" not in r.stdout
+    lines = r.stdout.splitlines()
+    k = lines.index("Combined Trajectory")
+    rows = [[float(t) for t in l.split()] for l in lines[k + 1:k + 4]]
+    allp = np.array(rows)
+    gaps = np.flatnonzero(np.isnan(allp[0]))
+    assert len(gaps) == 52 * 6 - 1
+    starts = allp[:, np.concatenate([[0], gaps + 1])].T
+    want = np.repeat(octet.checkpoints((6e-7, 1e-7, 1e-7, 1e-8)), 6, axis=0)
+    np.testing.assert_allclose(starts, want, rtol=1e-8, atol=1e-16)
+
