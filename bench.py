@@ -17,13 +17,15 @@ sample of the same workload on the host cores, and secondary objects:
 "slab" (the 1-D slab of 1e5 bundles that the reference runs on its CPU, in
 deterministic FP64, beside the whole solve on the CPU checker), "porous_film"
 (the periodic porous film with a cylindrical pore, 1e8 bundles, 64 x 64 map,
-with its own CPU baseline sample and FP32-vs-FP64 errors), "batched" (256
+with its own CPU baseline sample and FP32-vs-FP64 errors), "octet" (the
+reference's 42-cell octet-truss unit, FP32 and FP64, with a CPU sample),
+"batched" (256
 independent slab solves in one persistent launch against the same solves run
 one mcg_solve at a time) and "serving" (an open-loop Poisson request stream
 through the continuous-batching server, with p50/p99 latency).
 --headline-only skips the secondary objects and the CPU leg.
 
-    python bench.py --only slab|porous_film|batched|serving
+    python bench.py --only slab|porous_film|octet|batched|serving
 
 runs one secondary object alone (for profiling it).
 
@@ -83,12 +85,13 @@ def main():
                     default=os.path.join(ROOT, "profiles", "trajectory_kernel_bar1e7_fp32_traffic.json"),
                     help="counter-measured HBM bytes per launch of the kernel (rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE, scaled by the copy-probe calibration)")
-    ap.add_argument("--only", choices=("slab", "porous_film", "batched", "serving"),
+    ap.add_argument("--only", choices=("slab", "porous_film", "octet", "batched", "serving"),
                     help="run one secondary workload alone and print its JSON object")
     args = ap.parse_args()
     if args.only:
         run = {"slab": lambda: bench_slab(cpu=not args.no_cpu),
                "porous_film": lambda: bench_porous(cpu=not args.no_cpu),
+               "octet": lambda: bench_octet(cpu=not args.no_cpu),
                "batched": bench_batched, "serving": bench_serving}[args.only]
         print(json.dumps({args.only: run()}))
         return
@@ -172,12 +175,14 @@ def main():
         with open(args.traffic_json) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    batched = serving = porous = slab = None
+    batched = serving = porous = slab = octet_ = None
     if not args.headline_only:
         slab = bench_slab(cpu=not args.no_cpu)
         progress("slab done")
         porous = bench_porous(cpu=not args.no_cpu)
         progress("porous film done")
+        octet_ = bench_octet(cpu=not args.no_cpu)
+        progress("octet done")
         batched = bench_batched()
         progress("batched done")
         serving = bench_serving()
@@ -219,6 +224,7 @@ def main():
         "cpu_baseline": cpu,
         "slab": slab,
         "porous_film": porous,
+        "octet": octet_,
         "batched": batched,
         "serving": serving,
     }
@@ -310,6 +316,35 @@ def bench_slab(cpu=True, reps=20):
                                "kind": "port", "sample": "the whole 1e5-bundle solve, FP64",
                                "ms_per_solve": ct * 1e3}
         out["gpu_vs_cpu_tally_rel_diff"] = rel(f, fo)
+    return out
+
+
+def bench_octet(cpu=True, nbundles=10_000_000, cpu_sample=200_000):
+    """The reference's octet-truss unit (OctetDomain, 42 cells: 4-gon prisms
+    and pyramids, triangular prisms, boxes; 71 interface and 5 periodic
+    pairs), the struts' gradient source, maxscat 100: FP32 flight and the
+    deterministic FP64 mode on the GPU, and the CPU oracle on a sample."""
+    from montecarlocpp_amd import materials, octet
+    mat = materials.si_like(300.0)
+    dom = octet.octet((6e-7, 1e-7, 1e-7, 1e-8), (4, 2, 2, 1), 1.0)
+    out = {"workload": "octet truss unit s = 600 nm, d1 = d2 = 100 nm, t = 10 nm, div (4, 2, 2, 1), "
+                       "dT 1 K, Si-like at 300 K, 1e7 bundles, Multi tally, maxscat 100"}
+    with engine.Scene(mat, dom) as sc:
+        for prec, key in ((abi.FP32_FLIGHT, "fp32"), (abi.FP64, "fp64")):
+            prob = scenes.problem(nbundles, maxscat=100, seed=1, precision=prec)
+            sc.solve(scenes.problem(nbundles // 10, maxscat=100, seed=2, precision=prec))
+            f, st, _ = sc.solve(prob)
+            out[key] = {"scatter_steps": st["steps"], "kernel_ms": st["kernel_ms"],
+                        "steps_per_s": st["steps"] / (st["kernel_ms"] * 1e-3), "escapes": st["escapes"]}
+    if cpu:
+        from oracle import oracle  # CPU baseline leg only
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        prob = scenes.problem(nbundles, maxscat=100, seed=1, count=cpu_sample)
+        t0 = time.perf_counter()
+        _, cst, _ = oracle.solve(mat, dom, prob, threads=threads)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s", "cores": threads,
+                               "kind": "port", "sample": f"first {cpu_sample} of the 1e7 bundles, FP64"}
     return out
 
 
