@@ -264,9 +264,10 @@ int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob,
  * iteration i: step, hit (plane index, -1 intrinsic, -2 escape, -3 pore),
  * distance moved, position after the flight (x, y, z), direction, free path
  * left, mode and scatter count after the event, subdomain, fate (-1 while
- * alive), position after the event (x, y).  nrec receives the step count
- * (capped at cap). */
-#define MCG_TRACE_WIDTH 16
+ * alive), position after the event (x, y, z; a periodic face maps it,
+ * boundary.cpp:517-522).  nrec receives the step count (capped at cap; a
+ * last record with fate -1 means the trajectory went on past cap). */
+#define MCG_TRACE_WIDTH 17
 int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, double* rec,
               int64_t cap, int64_t* nrec);
 
@@ -279,10 +280,10 @@ int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, do
  * given one is normalised.  The start cell is the first subdomain that
  * contains pos (Domain::locate, domain.cpp:58-66); "Position not inside
  * domain" otherwise.  The draws use event 0 of bundle prob->first_bundle of
- * prob's seed; maxscat, maxloop and precision come from prob, which must be
- * a valid problem of the scene (so the domain needs a source, as for a
- * solve).  Records and nrec as mcg_trace; the fate of the last record tells
- * how it ended. */
+ * prob's seed; maxscat, maxloop and precision come from prob (nemit and
+ * the tally kind only need to be valid: the trajectory draws nothing from
+ * the emitters, so the domain needs no source).  Records and nrec as
+ * mcg_trace; the fate of the last record tells how it ended. */
 int mcg_trace_from(mcg_scene* scene, const mcg_problem_spec* prob, int64_t prop_cell,
                    const double pos[3], const double* dir, double* rec, int64_t cap, int64_t* nrec);
 

@@ -21,6 +21,8 @@ TEMP, FLUX, MULTI, CUMTEMP, CUMFLUX = range(5)
 FP64, FP32_FLIGHT = range(2)
 # bundle fates
 ABSORBED, ESCAPED, MAXSCAT, MAXLOOP = range(4)
+# doubles per trajectory record (mcg_trace / mcg_trace_from)
+TRACE_WIDTH = 17
 
 NFACE = {PARALLELEPIPED: 6, TRIPRISM: 5, TETRAHEDRON: 4}
 

@@ -255,14 +255,13 @@ int main(int argc, const char* argv[]) {
         uint64_t seed = se ? std::strtoull(se, nullptr, 10) : uint64_t(std::random_device()());
         Rng gen(seed);
         auto t0 = std::chrono::steady_clock::now();
+        std::cout << "  seeds: " << seed << "\n";
         if (nsim <= 1) {
-            std::cout << "Solution 0\n  seeds: " << seed << "\n";
             solveField(prob.get(), gen, &std::cout);
             const mcg_stats& st = prob->lastStats();
-            std::cout << "  key: " << prob->lastKey() << "  steps: " << st.steps
-                      << "  esc: " << st.escapes << "  kernel_ms: " << st.kernel_ms << "\n";
+            std::cout << "  steps: " << st.steps << "  esc: " << st.escapes << "  kernel_ms: " << st.kernel_ms
+                      << "\n";
         } else {
-            std::cout << "  seeds: " << seed << "\n";
             calcStats(prob.get(), nsim, gen, &std::cout);
         }
         double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

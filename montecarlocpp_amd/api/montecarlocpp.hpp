@@ -186,6 +186,8 @@ public:
         return mcg_material_spec{nw_, np_, disp_.data(), relax_.data(), T_};
     }
     double temp() const { return T_; }
+    long nw() const { return nw_; }
+    long np() const { return np_; }
     double cond() const { return tab_.cond; }
     double energySum() const { return tab_.energy_sum; }
     double fluxSum() const { return tab_.flux_sum; }
@@ -613,10 +615,13 @@ private:
 // and register them with addSdom in the reference's order.
 class Domain {
     Subdomain::Pointers sdomPtrs_;
-    std::vector<mcg_subdomain_spec> cells_;
-    std::vector<mcg_pair_spec> pairs_;
     std::vector<mcg_pore_spec> pores_;
-    mcg_domain_spec spec_{};
+    // the engine-layout description spec() assembles from the cells: a cache
+    // behind a const accessor, so a Domain is used through const pointers as
+    // the reference's problems hold it (problem.h:68)
+    mutable std::vector<mcg_subdomain_spec> cells_;
+    mutable std::vector<mcg_pair_spec> pairs_;
+    mutable mcg_domain_spec spec_{};
 
     int locate(const Boundary* b, int* face) const {
         for (size_t s = 0; s < sdomPtrs_.size(); ++s) {
@@ -645,7 +650,7 @@ public:
 
     // The scene in the engine's layout: cells in registration order, one pair
     // spec per makePair call.
-    const mcg_domain_spec& spec() {
+    const mcg_domain_spec& spec() const {
         cells_.clear();
         pairs_.clear();
         std::vector<std::pair<long, mcg_pair_spec>> ordered;
@@ -1330,7 +1335,7 @@ class Field {
     ArrayXXd data_;
 
 public:
-    Field(long rows, Domain* dom) {
+    Field(long rows, const Domain* dom) {
         const mcg_domain_spec& d = dom->spec();
         long cols = 0;
         for (int s = 0; s < d.nsdom; ++s) {
@@ -1383,13 +1388,13 @@ public:
 // ---------------------------------------------------------------- Problem
 class Problem {
     const Material* mat_;
-    Domain* dom_;
+    const Domain* dom_;
 
 public:
-    Problem(const Material* mat, Domain* dom) : mat_(mat), dom_(dom) {}
+    Problem(const Material* mat, const Domain* dom) : mat_(mat), dom_(dom) {}   // problem.h:76
     virtual ~Problem() = default;
     const Material* mat() const { return mat_; }
-    Domain* dom() const { return dom_; }
+    const Domain* dom() const { return dom_; }
     virtual Progress initProgress() const = 0;
     virtual ArrayXXd solve(Rng& gen, Progress* prog) const = 0;
 };
@@ -1397,7 +1402,9 @@ public:
 // TrajProblem(mat, dom, [prop,] pos, [dir,] maxscat, maxloop) with a given
 // start (problem.h:88-119, problem.cpp:163-299): one trajectory on the GPU
 // (mcg_trace_from).  solve() returns the points the bundle visits, 3 x n: the
-// start, then the end of every flight; hops() holds the step records.
+// start, then every position it takes (the end of each flight, and the
+// mapped position after a periodic face, as TrkPhonon records them); hops()
+// holds the step records.
 class TrajProblem : public Problem {
     mutable mcg_scene* scene_ = nullptr;
     long maxscat_, maxloop_;
@@ -1408,16 +1415,29 @@ class TrajProblem : public Problem {
     mutable std::vector<double> rec_;
     mutable uint64_t lastKey_ = 0;
 
+    static constexpr int64_t kMaxRecords = int64_t(1) << 20;   // steps one solve() records at most
+
+    static long propCell(const Material* mat, const Phonon::Prop& prop) {
+        if (prop.w() < 0 || prop.w() >= mat->nw() || prop.p() < 0 || prop.p() >= mat->np())
+            throw mcg::Error(MCG_EINVAL, "prop outside the material's table");
+        return prop.w() * mat->np() + prop.p();
+    }
+
 public:
-    TrajProblem(const Material* mat, Domain* dom, const Vector3d& pos, const Vector3d& dir,
+    TrajProblem(const Material* mat, const Domain* dom, const Phonon::Prop& prop, const Vector3d& pos,
+                const Vector3d& dir, long maxscat = 100, long maxloop = 0)
+        : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), prop_(propCell(mat, prop)), pos_(pos),
+          dir_(dir), hasDir_(true) {}
+    TrajProblem(const Material* mat, const Domain* dom, const Phonon::Prop& prop, const Vector3d& pos,
+                long maxscat = 100, long maxloop = 0)
+        : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), prop_(propCell(mat, prop)), pos_(pos),
+          hasDir_(false) {}
+    TrajProblem(const Material* mat, const Domain* dom, const Vector3d& pos, const Vector3d& dir,
                 long maxscat = 100, long maxloop = 0)
         : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), pos_(pos), dir_(dir), hasDir_(true) {}
-    TrajProblem(const Material* mat, Domain* dom, const Vector3d& pos, long maxscat = 100, long maxloop = 0)
+    TrajProblem(const Material* mat, const Domain* dom, const Vector3d& pos, long maxscat = 100,
+                long maxloop = 0)
         : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), pos_(pos), hasDir_(false) {}
-    TrajProblem(const Material* mat, Domain* dom, long propCell, const Vector3d& pos, const Vector3d& dir,
-                long maxscat = 100, long maxloop = 0)
-        : Problem(mat, dom), maxscat_(maxscat), maxloop_(maxloop), prop_(propCell), pos_(pos), dir_(dir),
-          hasDir_(true) {}
     TrajProblem(const TrajProblem&) = delete;
     ~TrajProblem() override {
         if (scene_) mcg_scene_destroy(scene_);
@@ -1438,17 +1458,38 @@ public:
         ps.maxloop = maxloop_;
         lastKey_ = gen();
         ps.seed = lastKey_;
-        const int64_t cap = std::max<long>(1, maxloop_ != 0 ? maxloop_ : 100 * maxscat_);
-        rec_.assign(size_t(std::min<int64_t>(cap, 1 << 20)) * MCG_TRACE_WIDTH, 0.0);
+        const int64_t loops = maxloop_ != 0 ? maxloop_ : 100 * maxscat_;
+        const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(loops, kMaxRecords));
+        rec_.assign(size_t(cap) * MCG_TRACE_WIDTH, 0.0);
         int64_t n = 0;
         const double p[3] = {pos_(0), pos_(1), pos_(2)}, d[3] = {dir_(0), dir_(1), dir_(2)};
-        mcg::check(mcg_trace_from(scene_, &ps, prop_, p, hasDir_ ? d : nullptr, rec_.data(),
-                                  int64_t(rec_.size() / MCG_TRACE_WIDTH), &n));
+        mcg::check(mcg_trace_from(scene_, &ps, prop_, p, hasDir_ ? d : nullptr, rec_.data(), cap, &n));
         rec_.resize(size_t(n) * MCG_TRACE_WIDTH);
-        ArrayXXd traj(3, n + 1);
-        for (int k = 0; k < 3; ++k) traj(k, 0) = pos_(k);
-        for (int64_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) traj(k, i + 1) = rec_[size_t(i) * MCG_TRACE_WIDTH + 3 + k];
+        if (n == cap && rec_[size_t(n - 1) * MCG_TRACE_WIDTH + 13] < 0.0)
+            throw mcg::Error(MCG_ERANGE, "trajectory longer than the record buffer (" +
+                                             std::to_string(kMaxRecords) + " steps)");
+        // the faces' kinds in the engine's plane order (cells in order, each
+        // cell's faces in container order)
+        const mcg_domain_spec& ds = dom()->spec();
+        std::vector<int> kind;
+        for (int si = 0; si < ds.nsdom; ++si)
+            for (int f = 0; f < ds.sdom[si].nface; ++f) kind.push_back(ds.sdom[si].face[f].kind);
+        // TrkPhonon records a point whenever the position changes
+        // (phonon.cpp:154-157): the end of every flight (Phonon::move), and
+        // the mapped position after a periodic face (boundary.cpp:519), which
+        // a flight that ended the bundle outside its cell never reaches
+        std::vector<double> pts{pos_(0), pos_(1), pos_(2)};
+        for (int64_t i = 0; i < n; ++i) {
+            const double* r = rec_.data() + size_t(i) * MCG_TRACE_WIDTH;
+            pts.insert(pts.end(), r + 3, r + 6);
+            const long hit = long(r[1]);
+            if (hit >= 0 && hit < long(kind.size()) && kind[size_t(hit)] == MCG_PERI && r[13] != double(MCG_ESCAPED))
+                pts.insert(pts.end(), r + 14, r + 17);
+        }
+        const long np = long(pts.size() / 3);
+        ArrayXXd traj(3, np);
+        for (long c = 0; c < np; ++c)
+            for (int k = 0; k < 3; ++k) traj(k, c) = pts[size_t(3 * c + k)];
         return traj;
     }
     // the step records of the last solve (MCG_TRACE_WIDTH doubles each)
@@ -1490,7 +1531,7 @@ protected:
     }
 
 public:
-    FieldProblem(const Material* mat, Domain* dom, int tally, long nemit, long maxscat,
+    FieldProblem(const Material* mat, const Domain* dom, int tally, long nemit, long maxscat,
                  long maxloop)
         : Problem(mat, dom), tally_(tally), nemit_(nemit), maxscat_(maxscat), maxloop_(maxloop) {}
     ~FieldProblem() override {
@@ -1526,25 +1567,25 @@ public:
 };
 
 struct TempProblem : FieldProblem {
-    TempProblem(const Material* m, Domain* d, long nemit, long maxscat, long maxloop = 0)
+    TempProblem(const Material* m, const Domain* d, long nemit, long maxscat, long maxloop = 0)
         : FieldProblem(m, d, MCG_TEMP, nemit, maxscat, maxloop) {}
 };
 struct FluxProblem : FieldProblem {
-    FluxProblem(const Material* m, Domain* d, long nemit, long maxscat, long maxloop = 0)
+    FluxProblem(const Material* m, const Domain* d, long nemit, long maxscat, long maxloop = 0)
         : FieldProblem(m, d, MCG_FLUX, nemit, maxscat, maxloop) {}
 };
 struct MultiProblem : FieldProblem {
-    MultiProblem(const Material* m, Domain* d, long nemit, long maxscat, long maxloop = 0)
+    MultiProblem(const Material* m, const Domain* d, long nemit, long maxscat, long maxloop = 0)
         : FieldProblem(m, d, MCG_MULTI, nemit, maxscat, maxloop) {}
 };
 struct CumTempProblem : FieldProblem {
-    CumTempProblem(const Material* m, Domain* d, long nemit, long size, long maxscat, long maxloop = 0)
+    CumTempProblem(const Material* m, const Domain* d, long nemit, long size, long maxscat, long maxloop = 0)
         : FieldProblem(m, d, MCG_CUMTEMP, nemit, maxscat, maxloop) {
         size_ = size;
     }
 };
 struct CumFluxProblem : FieldProblem {
-    CumFluxProblem(const Material* m, Domain* d, long nemit, long size, long maxscat, long maxloop = 0)
+    CumFluxProblem(const Material* m, const Domain* d, long nemit, long size, long maxscat, long maxloop = 0)
         : FieldProblem(m, d, MCG_CUMFLUX, nemit, maxscat, maxloop) {
         size_ = size;
     }
@@ -1565,11 +1606,17 @@ inline void printSolution(const ArrayXXd& sol, std::ostream& os = std::cout) {
 }
 
 // solveField (main.cpp:146-179): one GPU solve (the reference sums one
-// partial per OpenMP thread; the engine already covers every bundle).
+// partial per OpenMP thread; the engine already covers every bundle).  The
+// log gets the reference's "Solution n" and "Output" blocks; the line after
+// the header gives the solve's Philox key (where the reference prints its
+// threads' seeds).
 inline ArrayXXd solveField(const FieldProblem* prob, Rng& gen, std::ostream* log = nullptr) {
+    static long n = 0;
+    if (log) *log << "Solution " << n++ << "\n";
     Progress prog = prob->initProgress();
     ArrayXXd sol = prob->solve(gen, &prog);
     if (log) {
+        *log << "  key: " << prob->lastKey() << "\n";
         *log << "Output\n";
         printSolution(sol, *log);
     }

@@ -135,6 +135,12 @@ void rotation_to(const Vec& n_in, double* r) {
 
 uint32_t prob_to_thr(double p) { return p >= 1.0 ? 0xFFFFFFFFu : uint32_t(p * 4294967296.0); }
 
+// A partially specular wall draws specular when the event's word is below
+// p 2^32; its end points are the reference's own kinds, SpecBoundary at
+// p = 1 and DiffBoundary at p = 0 (boundary.cpp:283-287, 308-312), so a wall
+// there takes that kind and no draw can leave it.
+int partial_kind(double p) { return p >= 1.0 ? MCG_SPEC : (p <= 0.0 ? MCG_DIFF : MCG_PARTIAL); }
+
 struct FaceDef {
     Vec o, i, j;
     int shape;
@@ -535,6 +541,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             if (fs.kind == MCG_PARTIAL) {
                 if (!(fs.specularity >= 0.0 && fs.specularity <= 1.0))
                     return set_error(MCG_EINVAL, "specularity outside [0, 1]");
+                P.kind = partial_kind(fs.specularity);
                 P.spec_thr = prob_to_thr(fs.specularity);
             }
             s.planes.push_back(P);
@@ -666,7 +673,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
         P.r = ps.radius;
         P.r2 = ps.radius * ps.radius;
         P.eps = s.sdoms[ps.sdom].eps;
-        P.kind = ps.kind;
+        P.kind = ps.kind == MCG_PARTIAL ? partial_kind(ps.specularity) : ps.kind;
         P.spec_thr = ps.kind == MCG_PARTIAL ? prob_to_thr(ps.specularity)
                                             : (ps.kind == MCG_SPEC ? 0xFFFFFFFFu : 0u);
         // the pore runs through the cell's faces across its axis: a face
@@ -790,7 +797,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
     return MCG_OK;
 }
 
-int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem& p) {
+int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem& p, bool trace_only) {
     if (!spec) return set_error(MCG_EINVAL, "problem spec is NULL");
     if (spec->tally < MCG_TEMP || spec->tally > MCG_CUMFLUX) return set_error(MCG_EINVAL, "Invalid problem");
     if (spec->precision != MCG_FP64 && spec->precision != MCG_FP32_FLIGHT)
@@ -798,7 +805,10 @@ int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem&
     if (spec->nemit <= 0) return set_error(MCG_EINVAL, "nemit must be positive");
     if (spec->maxscat <= 0) return set_error(MCG_EINVAL, "maxscat must be positive");
     if (spec->maxloop < 0) return set_error(MCG_EINVAL, "maxloop must be non-negative");
-    if (s.emitters.empty()) return set_error(MCG_EINVAL, "No phonons to emit");
+    // maxloop_ = loopFactor_ * maxscat_ (problem.h:124) must stay an int64
+    if (spec->maxloop == 0 && spec->maxscat > INT64_MAX / 100)
+        return set_error(MCG_ERANGE, "maxscat too large for the default maxloop (100 maxscat)");
+    if (s.emitters.empty() && !trace_only) return set_error(MCG_EINVAL, "No phonons to emit");
     p.tally = spec->tally;
     p.precision = spec->precision;
     p.size = spec->size;
@@ -814,6 +824,18 @@ int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem&
         if (spec->size <= 0) return set_error(MCG_EINVAL, "size must be positive");
         p.step = (spec->maxscat - 1) / spec->size + (((spec->maxscat - 1) % spec->size) != 0 ? 1 : 0);
         if (p.step <= 0) return set_error(MCG_EINVAL, "maxscat must exceed 1 for a cumulative tally");
+    }
+    p.maxscat = spec->maxscat;
+    p.maxloop = spec->maxloop != 0 ? spec->maxloop : 100 * spec->maxscat;
+    p.seed = spec->seed;
+    if (trace_only) {
+        p.weight_sum = p.power = 0.0;
+        p.emit_count.clear();
+        p.emit_cdf.clear();
+        p.nemit = spec->nemit;
+        p.first = spec->first_bundle;
+        p.count = 1;
+        return MCG_OK;
     }
     double wsum = 0.0;
     for (double w : s.emit_weight) {
@@ -831,10 +853,7 @@ int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem&
         p.nemit += c;
         p.emit_cdf.push_back(p.nemit);
     }
-    p.maxscat = spec->maxscat;
-    p.maxloop = spec->maxloop != 0 ? spec->maxloop : 100 * spec->maxscat;
     p.power = wsum / double(p.nemit) * s.flux_sum / 4.0;
-    p.seed = spec->seed;
     p.first = spec->first_bundle;
     p.count = spec->bundle_count == 0 ? p.nemit - p.first : spec->bundle_count;
     if (p.first < 0 || p.count < 0 || p.first + p.count > p.nemit)
@@ -847,17 +866,21 @@ int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem&
 void finalise_field(const HostScene& s, const HostProblem& p, const double* direct,
                     const double* diff, double* out) {
     const int64_t R = p.rows, Cn = s.cols;
-    std::vector<double> sol(R * Cn);
     for (int64_t r = 0; r < R; ++r) {
         double run = 0.0;
         for (int64_t c = 0; c < Cn; ++c) {
             run += diff ? diff[r * (Cn + 1) + c] : 0.0;
-            sol[c * R + r] = direct[r * Cn + c] + run;
+            out[c * R + r] = direct[r * Cn + c] + run;
         }
     }
+    normalise_field(s, p, out);
+}
+
+void normalise_field(const HostScene& s, const HostProblem& p, double* sol) {
+    const int64_t R = p.rows, Cn = s.cols;
     switch (p.tally) {
         case MCG_TEMP:
-            for (double& v : sol) v = v / s.energy_sum;
+            for (int64_t i = 0; i < R * Cn; ++i) sol[i] = sol[i] / s.energy_sum;
             break;
         case MCG_MULTI:
             for (int64_t c = 0; c < Cn; ++c) sol[c * R] = sol[c * R] / s.energy_sum;
@@ -882,7 +905,7 @@ void finalise_field(const HostScene& s, const HostProblem& p, const double* dire
     for (int64_t c = 0; c < Cn; ++c)
         // a cell with no material (wholly inside a pore) reports zero
         for (int64_t r = 0; r < R; ++r)
-            out[c * R + r] = s.cell_vol[c] > 0.0 ? p.power * (sol[c * R + r] / s.cell_vol[c]) : 0.0;
+            sol[c * R + r] = s.cell_vol[c] > 0.0 ? p.power * (sol[c * R + r] / s.cell_vol[c]) : 0.0;
 }
 
 }  // namespace mcg

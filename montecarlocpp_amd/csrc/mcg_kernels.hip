@@ -987,6 +987,7 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
         r[13] = fate == 0xFFFFFFFFu ? -1.0 : double(fate);
         r[14] = double(L.pos.x);
         r[15] = double(L.pos.y);
+        r[16] = double(L.pos.z);
     }
     if (fate != 0xFFFFFFFFu && K.c_nscat) {
         const int64_t k = L.id - K.first;
@@ -1535,6 +1536,36 @@ __global__ void zero_kernel(double* a, size_t na, double* b, size_t nb, unsigned
     }
 }
 
+// Launch shape and LDS plan of a single solve, then the launch on `stream`
+// (K holds the solve's buffers; its LDS and tail fields are set here).
+template <typename F>
+int launch_solve(mcg_scene* sc, const HostProblem& p, KParams<F>& K, hipStream_t stream) {
+    const HostScene& h = sc->host;
+    const int rows = int(p.rows);
+    const size_t geo = lds_bytes<F>(h, rows, false, false);
+    const LdsPlan lp = plan_lds(geo, lds_bytes<F>(h, rows, true, false) - geo,
+                                lds_bytes<F>(h, rows, false, true) - geo, h.has_grid,
+                                sc->tune.lds_tables >= 0, sc->lds_limit);
+    const bool tables = lp.tables, tally = lp.tally;
+    const size_t lds = lp.bytes;
+    if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
+    K.lds_tally = tally ? 1 : 0;
+    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally, h.has_grid);
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
+    // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
+    const int block = sc->tune.block > 0 ? sc->tune.block : (blocks_per_cu == 1 ? 1024 : 512);
+    blocks_per_cu = std::min(blocks_per_cu, std::max(1, 2048 / block));
+    if (sc->tune.blocks_per_cu > 0) blocks_per_cu = std::min(blocks_per_cu, int(sc->tune.blocks_per_cu));
+    const int64_t needed = (p.count + block - 1) / block;
+    const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
+    K.tail = int64_t(MCG_TAIL_LANES) * grid * block;
+    void* args[] = {&K};
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, stream));
+    HIP_TRY(hipGetLastError());
+    return MCG_OK;
+}
+
 template <typename F>
 int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* counts, mcg_stats* stats,
               int64_t trace_id = -1, double* trace = nullptr, int64_t trace_cap = 0) {
@@ -1593,30 +1624,10 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
         K.trace = tbuf.p;
         K.trace_cap = trace_cap;
     }
-    const size_t geo = lds_bytes<F>(h, rows, false, false);
-    const LdsPlan lp = plan_lds(geo, lds_bytes<F>(h, rows, true, false) - geo,
-                                lds_bytes<F>(h, rows, false, true) - geo, h.has_grid,
-                                sc->tune.lds_tables >= 0, sc->lds_limit);
-    const bool tables = lp.tables, tally = lp.tally;
-    const size_t lds = lp.bytes;
-    if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
-    K.lds_tally = tally ? 1 : 0;
-    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally, h.has_grid);
-    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
-    // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
-    const int block = sc->tune.block > 0 ? sc->tune.block : (blocks_per_cu == 1 ? 1024 : 512);
-    blocks_per_cu = std::min(blocks_per_cu, std::max(1, 2048 / block));
-    if (sc->tune.blocks_per_cu > 0) blocks_per_cu = std::min(blocks_per_cu, int(sc->tune.blocks_per_cu));
-    const int64_t needed = (p.count + block - 1) / block;
-    const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
-    K.tail = int64_t(MCG_TAIL_LANES) * grid * block;
     if (!sc->ev0) HIP_TRY(hipEventCreate(&sc->ev0));
     if (!sc->ev1) HIP_TRY(hipEventCreate(&sc->ev1));
     HIP_TRY(hipEventRecord(sc->ev0, 0));
-    void* args[] = {&K};
-    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, 0));
-    HIP_TRY(hipGetLastError());
+    if ((rc = launch_solve<F>(sc, p, K, 0))) return rc;
     HIP_TRY(hipEventRecord(sc->ev1, 0));
     HIP_TRY(hipMemcpyAsync(sc->pinned, direct, nall * sizeof(double), hipMemcpyDeviceToHost, 0));
     HIP_TRY(hipStreamSynchronize(0));
@@ -1659,6 +1670,7 @@ int run_trace_from(mcg_scene* sc, const HostProblem& p, const TraceStart<F>& sta
     HIP_TRY(hipSetDevice(sc->device));
     const int rows = int(p.rows), cols = int(h.cols);
     if (p.rows > (1 << 20) || h.cols * p.rows > (1ll << 31)) return set_error(MCG_ERANGE, "tally too large");
+    if (p.maxloop <= 0) return set_error(MCG_EINVAL, "maxloop must be positive");
     const size_t nd = size_t(rows) * cols, nf = size_t(rows) * (cols + 1);
     DevBuf<double> tally, tbuf;
     DevBuf<uint32_t> cnt;
@@ -2419,7 +2431,7 @@ int mcg_trace_from(mcg_scene* scene, const mcg_problem_spec* prob, int64_t prop_
     one.first_bundle = 0;
     one.bundle_count = 1;
     mcg::HostProblem p;
-    int rc = mcg::build_problem(&one, h, p);
+    int rc = mcg::build_problem(&one, h, p, true);
     if (rc) return rc;
     p.first = prob->first_bundle;   // the stream's id; it need not be an emitted bundle
     int mode = -1;

@@ -140,10 +140,18 @@ struct HostProblem {
 // thread's last error message on failure
 int build_material(const mcg_material_spec* spec, HostScene& s);
 int build_domain(const mcg_domain_spec* spec, HostScene& s);
-int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem& p);
+// trace_only: a trajectory from a given start (TrajProblem with pos_ set,
+// problem.cpp:226-299) draws nothing from the emitters, so the domain needs
+// no source; nemit, counts and power are then left empty
+int build_problem(const mcg_problem_spec* spec, const HostScene& s, HostProblem& p, bool trace_only = false);
 int build_alias(const double* pdf, int64_t n, std::vector<uint64_t>& table);
+// out (rows x cols, column-major) = postProc of the raw sums, times power_
+// over the cell volumes (problem.cpp:439-444): direct is rows x cols and diff
+// rows x (cols + 1) (range adds, or NULL), both row-major
 void finalise_field(const HostScene& s, const HostProblem& p, const double* direct,
                     const double* diff, double* out);
+// the same from the folded raw sums already in sol (column-major), in place
+void normalise_field(const HostScene& s, const HostProblem& p, double* sol);
 
 int set_error(int code, const std::string& msg);
 

@@ -24,7 +24,7 @@ EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_pro
            "mcg_solve", "mcg_trace", "mcg_trace_from", "mcg_batch_solve", "mcg_batch_create", "mcg_batch_run",
            "mcg_batch_destroy")
 
-TRACE_WIDTH = 16
+TRACE_WIDTH = abi.TRACE_WIDTH
 
 _lib = None
 
@@ -202,7 +202,7 @@ class Scene:
         return out.reshape(info.cols, info.rows).T.copy(), st.as_dict(), cb
 
     def trace(self, prob, bundle, cap=4096):
-        """Step records of one bundle (mcg_trace): array (nsteps, 16)."""
+        """Step records of one bundle (mcg_trace): array (nsteps, TRACE_WIDTH)."""
         rec = np.zeros(cap * TRACE_WIDTH)
         n = C.c_int64()
         check(lib().mcg_trace(self.handle, C.byref(prob), int(bundle), abi.dptr(rec), cap,
@@ -211,7 +211,7 @@ class Scene:
 
     def trace_from(self, prob, pos, dir=None, prop_cell=-1, cap=4096):
         """Step records of one trajectory from a given start (mcg_trace_from,
-        TrajProblem with pos): array (nsteps, 16)."""
+        TrajProblem with pos): array (nsteps, TRACE_WIDTH)."""
         rec = np.zeros(cap * TRACE_WIDTH)
         n = C.c_int64()
         p = np.ascontiguousarray(pos, dtype=np.float64)

@@ -806,6 +806,10 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
                 double p = fs.specularity;
                 if (!(p >= 0.0 && p <= 1.0)) return fail(MCG_EINVAL, "specularity outside [0, 1]");
                 P.specThr = p >= 1.0 ? 0xFFFFFFFFu : uint32_t(p * 4294967296.0);
+                // the end points are SpecBoundary and DiffBoundary themselves
+                // (boundary.cpp:283-287, 308-312): no draw is taken there
+                if (p >= 1.0) P.kind = MCG_SPEC;
+                if (p <= 0.0) P.kind = MCG_DIFF;
             }
             P.emitter = emitting && (P.area * std::fabs(P.T) != 0.0);
             if (P.emitter) faceEmit[si].push_back(int(D.planes.size()));
@@ -897,6 +901,8 @@ int build_domain(const mcg_domain_spec* d, Domain& D) {
             if (!(ps.specularity >= 0.0 && ps.specularity <= 1.0))
                 return fail(MCG_EINVAL, "specularity outside [0, 1]");
             P.thr = ps.specularity >= 1.0 ? 0xFFFFFFFFu : uint32_t(ps.specularity * 4294967296.0);
+            if (ps.specularity >= 1.0) P.kind = MCG_SPEC;
+            if (ps.specularity <= 0.0) P.kind = MCG_DIFF;
         }
         // the pore runs through the cell's faces across its axis: a face
         // there keeps the hole (Peri back onto this cell) or closes it; an
@@ -974,13 +980,19 @@ struct Problem {
     int64_t first, count;
 };
 
-int build_problem(const mcg_problem_spec* p, const Material& M, const Domain& D, Problem& P) {
+// trace_only: a trajectory from a given start (TrajProblem with pos_ set,
+// problem.cpp:226-299) never touches the emitters, so a domain without a
+// source is accepted and nemit, counts and power stay empty
+int build_problem(const mcg_problem_spec* p, const Material& M, const Domain& D, Problem& P,
+                  bool trace_only = false) {
     if (!p) return fail(MCG_EINVAL, "problem spec is NULL");
     if (p->tally < MCG_TEMP || p->tally > MCG_CUMFLUX) return fail(MCG_EINVAL, "Invalid problem");
     if (p->nemit <= 0) return fail(MCG_EINVAL, "nemit must be positive");
     if (p->maxscat <= 0) return fail(MCG_EINVAL, "maxscat must be positive");
     if (p->maxloop < 0) return fail(MCG_EINVAL, "maxloop must be non-negative");
-    if (D.emitters.empty()) return fail(MCG_EINVAL, "No phonons to emit");
+    if (p->maxloop == 0 && p->maxscat > INT64_MAX / 100)
+        return fail(MCG_ERANGE, "maxscat too large for the default maxloop (100 maxscat)");
+    if (D.emitters.empty() && !trace_only) return fail(MCG_EINVAL, "No phonons to emit");
     P.tally = p->tally;
     P.size = p->size;
     switch (p->tally) {  // problem.cpp:468-471, 501-504, 534-537, 576-579, 624-627
@@ -997,6 +1009,18 @@ int build_problem(const mcg_problem_spec* p, const Material& M, const Domain& D,
         P.step = (p->maxscat - 1) / p->size;
         if ((p->maxscat - 1) % p->size != 0) P.step++;
         if (P.step <= 0) return fail(MCG_EINVAL, "maxscat must exceed 1 for a cumulative tally");
+    }
+    P.maxscat = p->maxscat;
+    P.maxloop = p->maxloop != 0 ? p->maxloop : 100 * p->maxscat;
+    P.seed = p->seed;
+    if (trace_only) {
+        P.weightSum = P.power = 0.0;
+        P.emitPdf.clear();
+        P.emitCdf.clear();
+        P.nemit = p->nemit;
+        P.first = p->first_bundle;
+        P.count = 1;
+        return MCG_OK;
     }
     // problem.cpp:319-341
     double wsum = 0.0;
@@ -1017,10 +1041,7 @@ int build_problem(const mcg_problem_spec* p, const Material& M, const Domain& D,
         P.nemit += c;
         P.emitCdf.push_back(P.nemit);
     }
-    P.maxscat = p->maxscat;
-    P.maxloop = p->maxloop != 0 ? p->maxloop : 100 * p->maxscat;
     P.power = wsum / double(P.nemit) * M.fluxSum / 4.0;
-    P.seed = p->seed;
     P.first = p->first_bundle;
     P.count = p->bundle_count == 0 ? P.nemit - P.first : p->bundle_count;
     if (P.first < 0 || P.count < 0 || P.first + P.count > P.nemit)
@@ -1036,7 +1057,16 @@ struct Phonon {
     V3 pos, dir;
     double scatNext;
     int64_t nscat;
+    // TrkPhonon (phonon.cpp:133-157): when set, every new position is
+    // appended, as its pos(newPos) does
+    std::vector<V3>* track;
 };
+
+// Phonon::pos(newPos) / TrkPhonon::pos(newPos) (phonon.cpp:154-157)
+inline void set_pos(Phonon& ph, const V3& p) {
+    ph.pos = p;
+    if (ph.track) ph.track->push_back(p);
+}
 
 // random.cpp:16-27
 V3 draw_iso(EventRng& g) {
@@ -1210,7 +1240,7 @@ int advect(const Domain& D, const Sdom& S, Phonon& ph, double& dist) {
     }
     ph.scatNext -= minDistance;
     if (ph.scatNext < DBL_MIN) ph.scatNext = 0.0;
-    ph.pos = add(ph.pos, scale(minDistance, ph.dir));
+    set_pos(ph, add(ph.pos, scale(minDistance, ph.dir)));   // Phonon::move, phonon.cpp:95-105
     dist = minDistance;
     bool negDist = minDistance < -S.eps;
     bool outside = !is_inside(D, S, ph.pos);
@@ -1366,6 +1396,7 @@ void record(double* trace, int64_t cap, uint32_t nstep, int hit, double dist, V3
     r[13] = fate;
     r[14] = ph.pos.x;
     r[15] = ph.pos.y;
+    r[16] = ph.pos.z;
 }
 
 // The step loop of FieldProblem::solve (problem.cpp:401-437) for bundle n
@@ -1475,7 +1506,7 @@ void run_phonon(const Material& M, const Domain& D, const Problem& P, int64_t n,
                     sdom = B.sdom;
                     break;
                 case MCG_PERI: {  // boundary.cpp:517-522
-                    ph.pos = add(mul(B.periRot, ph.pos), B.periT);
+                    set_pos(ph, add(mul(B.periRot, ph.pos), B.periT));
                     ph.dir = normalized(mul(B.periRot, ph.dir));
                     sdom = D.planes[B.pair].sdom;
                     break;
@@ -1576,12 +1607,12 @@ struct Built {
 };
 
 int build_all(const mcg_material_spec* mat, const mcg_domain_spec* dom,
-              const mcg_problem_spec* prob, Built& B) {
+              const mcg_problem_spec* prob, Built& B, bool trace_only = false) {
     int rc = build_material(mat, B.M);
     if (rc) return rc;
     rc = build_domain(dom, B.D);
     if (rc) return rc;
-    return build_problem(prob, B.M, B.D, B.P);
+    return build_problem(prob, B.M, B.D, B.P, trace_only);
 }
 
 }  // namespace
@@ -1754,17 +1785,19 @@ int oracle_trace(const mcg_material_spec* mat, const mcg_domain_spec* dom,
 // the mode from the scattering distribution unless given (drawScatProp),
 // an isotropic direction unless given (drawIso), the first free path, all
 // from event 0 of bundle prob->first_bundle; the start cell by
-// Domain::locate (domain.cpp:58-66).
-int oracle_trace_from(const mcg_material_spec* mat, const mcg_domain_spec* dom,
-                      const mcg_problem_spec* prob, int64_t prop_cell, const double pos[3],
-                      const double* dir, double* rec, int64_t cap, int64_t* nrec) {
-    if (!prob || !pos || !rec || !nrec || cap <= 0) return fail(MCG_EINVAL, "NULL argument");
+// Domain::locate (domain.cpp:58-66).  The domain needs no source.  With
+// track set the trajectory's points are collected as TrkPhonon does: the
+// start, then every position the bundle takes (phonon.cpp:133-157).
+static int trace_from(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                      const mcg_problem_spec* prob, int64_t prop_cell, const double pos[3], const double* dir, double* rec, int64_t cap,
+               int64_t* nrec, std::vector<V3>* track) {
+    if (!prob || !pos) return fail(MCG_EINVAL, "NULL argument");
     if (prob->first_bundle < 0) return fail(MCG_ERANGE, "bundle id must be non-negative");
     mcg_problem_spec one = *prob;
     one.first_bundle = 0;
     one.bundle_count = 1;
     Built B;
-    int rc = build_all(mat, dom, &one, B);
+    int rc = build_all(mat, dom, &one, B, true);
     if (rc) return rc;
     B.P.first = prob->first_bundle;   // the stream's id; it need not be an emitted bundle
     int mode = -1;
@@ -1790,13 +1823,41 @@ int oracle_trace_from(const mcg_material_spec* mat, const mcg_domain_spec* dom,
     ph.sign = 1;
     ph.scatNext = 0.0;
     ph.nscat = 0;
+    ph.track = track;
+    if (track) track->push_back(p);
     draw_scat_next(B.M, ph, g);
     Field F{B.P.rows, B.D.cols, std::vector<double>(B.P.rows * B.D.cols, 0.0)};
     Counters C;
     uint32_t ns, nb, nt, fa;
     mcg_counts counts{&ns, &nb, &nt, &fa};
     run_phonon(B.M, B.D, B.P, n, ph, sdom, F, C, &counts, rec, cap);
-    *nrec = std::min<int64_t>(nt, cap);
+    if (nrec) *nrec = std::min<int64_t>(nt, cap);
+    return MCG_OK;
+}
+
+int oracle_trace_from(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                      const mcg_problem_spec* prob, int64_t prop_cell, const double pos[3],
+                      const double* dir, double* rec, int64_t cap, int64_t* nrec) {
+    if (!rec || !nrec || cap <= 0) return fail(MCG_EINVAL, "NULL argument");
+    return trace_from(mat, dom, prob, prop_cell, pos, dir, rec, cap, nrec, nullptr);
+}
+
+// The 3 x n points TrajProblem::solve returns (phn.trajectory(),
+// phonon.cpp:159-169), column-major into pts (cap points at most); *npts
+// receives the full count.
+int oracle_trajectory_from(const mcg_material_spec* mat, const mcg_domain_spec* dom,
+                           const mcg_problem_spec* prob, int64_t prop_cell, const double pos[3],
+                           const double* dir, double* pts, int64_t cap, int64_t* npts) {
+    if (!pts || !npts || cap <= 0) return fail(MCG_EINVAL, "NULL argument");
+    std::vector<V3> track;
+    int rc = trace_from(mat, dom, prob, prop_cell, pos, dir, nullptr, 0, nullptr, &track);
+    if (rc) return rc;
+    *npts = int64_t(track.size());
+    for (int64_t i = 0; i < *npts && i < cap; ++i) {
+        pts[3 * i] = track[size_t(i)].x;
+        pts[3 * i + 1] = track[size_t(i)].y;
+        pts[3 * i + 2] = track[size_t(i)].z;
+    }
     return MCG_OK;
 }
 

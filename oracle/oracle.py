@@ -60,6 +60,10 @@ def lib():
                                         C.POINTER(abi.ProblemSpec), C.c_int64, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
                                         C.POINTER(C.c_int64)]
+        L.oracle_trajectory_from.argtypes = [C.POINTER(abi.MaterialSpec), C.POINTER(abi.DomainSpec),
+                                             C.POINTER(abi.ProblemSpec), C.c_int64, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                                             C.POINTER(C.c_int64)]
         L.oracle_math.argtypes = [C.c_int32, C.POINTER(C.c_double), C.c_int64,
                                   C.POINTER(C.c_double)]
         L.oracle_math.restype = None
@@ -161,11 +165,11 @@ def sample_emission(mat, dom, prob, n):
 
 
 def trace(mat, dom, prob, bundle, cap=4096):
-    rec = np.zeros(cap * 16)
+    rec = np.zeros(cap * abi.TRACE_WIDTH)
     n = C.c_int64()
     _check(lib().oracle_trace(C.byref(mat.spec), C.byref(dom.spec()), C.byref(prob), int(bundle),
                               abi.dptr(rec), cap, C.byref(n)))
-    return rec.reshape(cap, 16)[:n.value].copy()
+    return rec.reshape(cap, abi.TRACE_WIDTH)[:n.value].copy()
 
 
 def math_eval(fn, x):
@@ -187,12 +191,26 @@ def disk_area(q):
 
 def trace_from(mat, dom, prob, pos, dir=None, prop_cell=-1, cap=4096):
     """TrajProblem with a given start: the checker's records, as engine.Scene.trace_from."""
-    rec = np.zeros(cap * 16)
+    rec = np.zeros(cap * abi.TRACE_WIDTH)
     n = C.c_int64()
     p = np.ascontiguousarray(pos, dtype=np.float64)
     d = None if dir is None else np.ascontiguousarray(dir, dtype=np.float64)
     _check(lib().oracle_trace_from(C.byref(mat.spec), C.byref(dom.spec()), C.byref(prob), int(prop_cell),
                                    abi.dptr(p), None if d is None else abi.dptr(d), abi.dptr(rec), cap,
                                    C.byref(n)))
-    return rec.reshape(cap, 16)[:n.value].copy()
+    return rec.reshape(cap, abi.TRACE_WIDTH)[:n.value].copy()
+
+
+def trajectory_from(mat, dom, prob, pos, dir=None, prop_cell=-1, cap=1 << 16):
+    """The points TrajProblem::solve returns for a given start, collected as
+    TrkPhonon does (the start, then every new position): array (3, n)."""
+    pts = np.zeros(3 * cap)
+    n = C.c_int64()
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    d = None if dir is None else np.ascontiguousarray(dir, dtype=np.float64)
+    _check(lib().oracle_trajectory_from(C.byref(mat.spec), C.byref(dom.spec()), C.byref(prob), int(prop_cell),
+                                        abi.dptr(p), None if d is None else abi.dptr(d), abi.dptr(pts), cap,
+                                        C.byref(n)))
+    assert n.value <= cap, "trajectory longer than the point buffer"
+    return pts[:3 * n.value].reshape(n.value, 3).T.copy()
 
