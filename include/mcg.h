@@ -258,6 +258,23 @@ int mcg_scene_plan(const mcg_scene* scene, const mcg_problem_spec* prob, mcg_pro
 int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob,
               double* out, int64_t out_len, mcg_counts* counts, mcg_stats* stats);
 
+/* The solve that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is
+ * mcg_solve without per-bundle counts, on a scene already created: every
+ * per-solve device buffer (tallies, the range-add array, counters, emitter
+ * counts, the folded output) lives in one static device array of
+ * MCG_STATIC_SCRATCH_BYTES on each GPU, held until the process ends, and no
+ * hipMalloc, hipFree or hipHostMalloc is called.  A problem whose buffers do
+ * not fit (rows x cols of about 2^18 doubles at most) is refused with
+ * MCG_ERANGE.  Every call on a GPU shares that GPU's array and clears what it
+ * uses: calls that overlap on one GPU (two host threads, or any use while a
+ * call is in progress) race and give wrong results.  stats.kernel_ms is the
+ * host-measured time from launch to completion. */
+#define MCG_STATIC_SCRATCH_BYTES (8u * 1024u * 1024u)
+int mcg_solve_static(mcg_scene* scene, const mcg_problem_spec* prob, double* out, int64_t out_len,
+                     mcg_stats* stats);
+
 /* Trajectory of one bundle of a field solve, step by step: the GPU
  * counterpart of TrajProblem::solve (problem.cpp:226-299), which prints the
  * hops of one bundle.  Record i (MCG_TRACE_WIDTH doubles) describes loop

@@ -21,11 +21,19 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "mcg_device.h"
 #include "mcg_scene.h"
+
+// Device memory of mcg_solve_static, the solve that allocates nothing: every
+// per-solve buffer of that entry point lives here.  Each GPU the library runs
+// on holds its own copy until the process ends.  In host code this name is a
+// stand-in, not a device address: the host side takes each device's address
+// with hipGetSymbolAddress once (static_scratch below) and uses only that.
+static __device__ __align__(256) unsigned char scratch[MCG_STATIC_SCRATCH_BYTES];
 
 namespace mcg {
 
@@ -1661,6 +1669,108 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     return MCG_OK;
 }
 
+// ------------------------------------------------- the solve that allocates nothing
+// Resolves the range adds into the solve's output layout: sol (rows x cols,
+// column-major) = direct + the running sum of the difference array along
+// each row (both row-major).  One wavefront per row, the row scanned in
+// tiles of 64 columns.
+__global__ void __launch_bounds__(64) fold_kernel(const double* direct, const double* diff, int rows, int cols,
+                                                  double* sol) {
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    double carry = 0.0;
+    for (int c0 = 0; c0 < cols; c0 += kWave) {
+        const int c = c0 + lane;
+        double v = c < cols ? diff[size_t(r) * (cols + 1) + c] : 0.0;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double u = __shfl_up(v, off);
+            if (lane >= off) v += u;
+        }
+        if (c < cols) sol[size_t(c) * rows + r] = direct[size_t(r) * cols + c] + (carry + v);
+        carry += __shfl(v, kWave - 1);
+    }
+}
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxStaticDevices] = {};
+
+// The static scratch of the current device (`dev`), its address taken once
+// per device.
+int static_scratch(int dev, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxStaticDevices) return set_error(MCG_EDEVICE, "device index beyond the scratch table");
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(MCG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return MCG_OK;
+}
+
+// mcg_solve with every device buffer in the static scratch:
+// [direct rows x cols | diff rows x (cols + 1) | counters | output rows x cols
+// | emitter cumulative counts], 8-byte words.
+template <typename F>
+int run_solve_static(mcg_scene* sc, const HostProblem& p, double* out, mcg_stats* stats) {
+    const HostScene& h = sc->host;
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(sc->device));
+    if (p.maxloop <= 0) return set_error(MCG_EINVAL, "maxloop must be positive");
+    const int rows = int(p.rows), cols = int(h.cols);
+    const size_t nd = size_t(p.rows) * size_t(h.cols), nf = size_t(p.rows) * size_t(h.cols + 1);
+    const size_t ncdf = p.emit_cdf.size();
+    const size_t words = nd + nf + C_COUNT + nd + ncdf;
+    if (p.rows > (1 << 20) || words > MCG_STATIC_SCRATCH_BYTES / 8)
+        return set_error(MCG_ERANGE, "the solve needs " + std::to_string(words * 8) +
+                                         " bytes of device scratch; mcg_solve_static has " +
+                                         std::to_string(size_t(MCG_STATIC_SCRATCH_BYTES)));
+    unsigned char* base = nullptr;
+    int rc = static_scratch(sc->device, &base);
+    if (rc) return rc;
+    double* const direct = reinterpret_cast<double*>(base);
+    double* const diff = direct + nd;
+    unsigned long long* const ctr = reinterpret_cast<unsigned long long*>(diff + nf);
+    double* const sol = reinterpret_cast<double*>(ctr + C_COUNT);
+    int64_t* const cdf = reinterpret_cast<int64_t*>(sol + nd);
+    HIP_TRY(hipMemcpy(cdf, p.emit_cdf.data(), ncdf * sizeof(int64_t), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(zero_kernel, dim3(int(std::min<size_t>(1024, (nd + nf + 255) / 256 + 1))), dim3(256), 0, 0,
+                       direct, nd + nf, static_cast<double*>(nullptr), size_t(0), ctr, size_t(C_COUNT));
+    HIP_TRY(hipGetLastError());
+    KParams<F> K{};
+    fill_params<F>(sc, p, K);
+    K.emit_cdf = cdf;
+    K.direct = direct;
+    K.diff = diff;
+    K.counters = ctr;
+    K.next = ctr + C_NEXT;
+    const auto tk = std::chrono::steady_clock::now();
+    if ((rc = launch_solve<F>(sc, p, K, 0))) return rc;
+    hipLaunchKernelGGL(fold_kernel, dim3(rows), dim3(kWave), 0, 0, direct, diff, rows, cols, sol);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(0));
+    const auto tk1 = std::chrono::steady_clock::now();
+    unsigned long long hc[C_COUNT];
+    HIP_TRY(hipMemcpy(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out, sol, nd * sizeof(double), hipMemcpyDeviceToHost));
+    normalise_field(h, p, out);
+    if (stats) {
+        stats->bundles = int64_t(hc[C_BUNDLES]);
+        stats->steps = int64_t(hc[C_STEPS]);
+        stats->escapes = int64_t(hc[C_ESC]);
+        stats->absorbed = int64_t(hc[C_ABS]);
+        stats->truncated = int64_t(hc[C_TRUNC]);
+        stats->kernel_ms = std::chrono::duration<double, std::milli>(tk1 - tk).count();
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (int64_t(hc[C_BUNDLES]) != p.count)
+        return set_error(MCG_EDEVICE, "kernel finished with bundles unaccounted for");
+    return MCG_OK;
+}
+
 // Host side of mcg_trace_from: buffers for the one lane's tallies (kept on
 // the GPU and dropped), its counts and its records.
 template <typename F>
@@ -2398,6 +2508,23 @@ int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob, double* out, int64
     }
     if (p.precision == MCG_FP64) return mcg::run_solve<double>(scene, p, out, counts, stats);
     return mcg::run_solve<float>(scene, p, out, counts, stats);
+}
+
+int mcg_solve_static(mcg_scene* scene, const mcg_problem_spec* prob, double* out, int64_t out_len,
+                     mcg_stats* stats) {
+    if (!scene || !prob || !out) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    mcg::HostProblem p;
+    int rc = mcg::build_problem(prob, scene->host, p);
+    if (rc) return rc;
+    if (out_len < p.rows * scene->host.cols) return mcg::set_error(MCG_ERANGE, "output buffer too small");
+    if (p.count == 0) {
+        for (int64_t i = 0; i < p.rows * scene->host.cols; ++i) out[i] = 0.0;
+        mcg::normalise_field(scene->host, p, out);
+        if (stats) *stats = mcg_stats{};
+        return MCG_OK;
+    }
+    if (p.precision == MCG_FP64) return mcg::run_solve_static<double>(scene, p, out, stats);
+    return mcg::run_solve_static<float>(scene, p, out, stats);
 }
 
 int mcg_trace(mcg_scene* scene, const mcg_problem_spec* prob, int64_t bundle, double* rec,

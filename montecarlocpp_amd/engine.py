@@ -22,7 +22,7 @@ EXPORTS = ("mcg_abi_version", "mcg_last_error", "mcg_material_compute", "mcg_pro
            "mcg_server_poll",
            "mcg_server_cancel", "mcg_server_info", "mcg_server_destroy",
            "mcg_solve", "mcg_trace", "mcg_trace_from", "mcg_batch_solve", "mcg_batch_create", "mcg_batch_run",
-           "mcg_batch_destroy")
+           "mcg_batch_destroy", "mcg_solve_static")
 
 TRACE_WIDTH = abi.TRACE_WIDTH
 
@@ -76,6 +76,8 @@ def lib():
                                          C.POINTER(abi.Tune), C.POINTER(C.c_double)]
         L.mcg_solve.argtypes = [C.c_void_p, PS, C.POINTER(C.c_double), C.c_int64,
                                 C.POINTER(abi.Counts), C.POINTER(abi.Stats)]
+        L.mcg_solve_static.argtypes = [C.c_void_p, PS, C.POINTER(C.c_double), C.c_int64,
+                                       C.POINTER(abi.Stats)]
         L.mcg_trace.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double), C.c_int64,
                                 C.POINTER(C.c_int64)]
         L.mcg_trace_from.argtypes = [C.c_void_p, PS, C.c_int64, C.POINTER(C.c_double),
