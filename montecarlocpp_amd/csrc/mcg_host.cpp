@@ -424,6 +424,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
     s.inter_list.clear();
     s.pores.clear();
     s.has_1d = s.has_grid = false;
+    s.grid_2d = true;
     const int ns = spec->nsdom;
     std::vector<double> face_temp;            // pre-pairing temperature per plane
     std::vector<double> face_area;
@@ -494,6 +495,10 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
         } else {
             d.accum = positive + 1;
             s.has_grid = true;
+            s.grid_2d = s.grid_2d && positive == 2;
+            int g = 0;
+            for (int k = 0; k < 3; ++k)
+                if (div[k] > 0 && g < 2) d.gax[g++] = k;
         }
         for (int k = 0; k < 3; ++k) {
             if (shapes[si][k] > (1ll << 30)) return set_error(MCG_ERANGE, "too many tally divisions");

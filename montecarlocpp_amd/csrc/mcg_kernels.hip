@@ -23,6 +23,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mcg_device.h"
@@ -385,14 +386,83 @@ __device__ __forceinline__ bool walk_next(Walk<F>& W, const Sink& sink) {
     return false;
 }
 
+// The deferred 2-D / 3-D walk of FP32 flight (NA = 2 grid axes when every
+// grid of the scene is 2-D, else 3): the crossings taken one per call, the
+// axis whose next crossing comes first advancing (an axis with none left
+// holds parameter 2).  Crossings of two axes at the same parameter are taken
+// one after the other, the first with a zero-length share; the statistical
+// comparison of FP32 flight does not see the difference from the merged map
+// entry of field.cpp:156-218, which FP64 keeps.  The amounts are FP64 (exact:
+// a sign times an FP32 value), so a share costs one FP64 multiply; the row
+// stride and row count ride in the walk's registers, so the loop reads no
+// kernel parameter.
+template <int NA>
+struct WalkN {
+    float t[NA];             // next crossing parameter per axis, 2 once none is left
+    float dt[NA];            // parameter step between the axis's crossings
+    int32_t n[NA];           // crossings left per axis
+    int32_t dcol[NA];        // column step of a crossing
+    int32_t idx;             // flat index of the current cell's first row
+    int32_t rstride;         // row stride (the tally's columns)
+    int32_t nv;              // amounts (rows) of the step
+    float prev;              // parameter reached
+    double amt[kMaxRows];
+};
+
+// One map entry of the deferred walk; true after the last segment.
+template <int NA>
+__device__ __forceinline__ bool walk_next(WalkN<NA>& W, const Sink& sink) {
+    // the parameters are ordinary numbers: a compare and select, without the
+    // NaN canonicalisation fminf costs
+    float tm = W.t[0];
+#pragma unroll
+    for (int q = 1; q < NA; ++q) tm = W.t[q] < tm ? W.t[q] : tm;
+    const bool done = tm == 2.0f;
+    if (!done || W.prev != 1.0f) {
+        const double seg = double((done ? 1.0f : tm) - W.prev);
+        double* p = sink.direct + W.idx;
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j) {
+            if (j < W.nv) unsafeAtomicAdd(p, W.amt[j] * seg);
+            p += W.rstride;
+        }
+    }
+    if (done) return true;
+    W.prev = tm;
+    bool open = true;   // no axis advanced yet
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+        const bool take = open && W.t[q] == tm;
+        open = open && !take;
+        if (take) {
+            W.idx += W.dcol[q];
+            W.n[q] -= 1;
+            W.t[q] = W.n[q] > 0 ? W.t[q] + W.dt[q] : 2.0f;
+        }
+    }
+    return false;
+}
+
+template <typename F, int kDefer>
+struct WalkSel {
+    using type = WalkN<kDefer>;
+};
+template <typename F>
+struct WalkSel<F, 0> {
+    using type = Walk<F>;
+};
+template <typename F, int kDefer>
+using WalkOf = typename WalkSel<F, kDefer>::type;
+
 // Field::accumulate (field.cpp:92-220) of the amounts `famt` (nv entries on
-// rows r0..) over the segment a -> b of subdomain S.  With kDefer a 2-D /
-// 3-D walk is only set up in W (the caller runs walk_next later) and the
-// function returns true; otherwise everything is added before it returns.
-template <typename F, bool kDefer>
+// rows r0..) over the segment a -> b of subdomain S.  With kDefer (FP32
+// flight) a 2-D / 3-D walk is only set up in W (the caller runs walk_next
+// later) and the function returns true; otherwise everything is added before
+// it returns.
+template <typename F, int kDefer>
 __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink,
                                            const dev::V3<F>& a, const dev::V3<F>& b, int r0,
-                                           int nv, const F (&famt)[kMaxRows], Walk<F>& W) {
+                                           int nv, const F (&famt)[kMaxRows], WalkOf<F, kDefer>& W) {
     double amt[kMaxRows];
 #pragma unroll
     for (int j = 0; j < kMaxRows; ++j) amt[j] = double(famt[j]);
@@ -465,6 +535,37 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         bi[k] = min(max(vb, 0), S.max[k]);
         ei[k] = min(max(ve, 0), S.max[k]);
     }
+    if constexpr (kDefer != 0) {
+        static_assert(sizeof(F) == 4, "deferred walks are an FP32 flight path");
+        // the crossings of each walked axis: first parameter, step, count
+        int col = S.col_off;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) col += bi[k] * S.stride[k];
+#pragma unroll
+        for (int q = 0; q < kDefer; ++q) {
+            const int k = kDefer == 3 ? q : S.gax[q];
+            const F b0 = k == 0 ? bc[0] : (k == 1 ? bc[1] : bc[2]);
+            const F e0 = k == 0 ? ec[0] : (k == 1 ? ec[1] : ec[2]);
+            const int bk = k == 0 ? bi[0] : (k == 1 ? bi[1] : bi[2]);
+            const int ek = k == 0 ? ei[0] : (k == 1 ? ei[1] : ei[2]);
+            const int sk = k == 0 ? S.stride[0] : (k == 1 ? S.stride[1] : S.stride[2]);
+            const F dc = e0 - b0;
+            const bool none = fabsf(dc) < FLT_MIN || bk == ek;
+            const F r = __builtin_amdgcn_rcpf(dc);
+            const bool up = bk < ek;
+            W.t[q] = none ? 2.0f : (F(up ? bk + 1 : bk) - b0) * r;
+            W.dt[q] = fabsf(r);
+            W.n[q] = none ? 0 : (up ? ek - bk : bk - ek);
+            W.dcol[q] = up ? sk : -sk;
+        }
+        W.idx = r0 * sink.cols + col;
+        W.rstride = sink.cols;
+        W.nv = nv;
+        W.prev = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j) W.amt[j] = double(famt[j]);
+        return true;
+    } else {
     // 2-D / 3-D: the crossings in parameter order (walk_next)
     W.col = S.col_off;
 #pragma unroll
@@ -499,10 +600,10 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
     W.nv = nv;
 #pragma unroll
     for (int j = 0; j < kMaxRows; ++j) W.amt[j] = famt[j];
-    if (kDefer) return true;
     while (!walk_next<F>(W, sink)) {
     }
     return false;
+    }
 }
 
 // Scatter off a cylindrical pore wall.  The wall normal at the hit point
@@ -537,6 +638,12 @@ __device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Eve
                     txz + twy, tyz - twx, F(1) - (txx + tyy)};
     L.dir = dev::normalized(dev::matvec(r, dev::draw_aniso<F>(b0, b1, false)));
     L.nscat++;
+}
+
+// Lanes set in a wavefront mask, as a 32-bit count (scalar instructions;
+// a 64-bit count would be compared on the vector unit)
+__device__ __forceinline__ int popc(uint64_t m) {
+    return __builtin_popcount(uint32_t(m)) + __builtin_popcount(uint32_t(m >> 32));
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -722,9 +829,9 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
 // boundary or intrinsic scatter, termination test.  Returns the bundle's fate
 // when it ends, 0xFFFFFFFF while it lives.  `walking` is set when the step's
 // 2-D / 3-D tally walk was deferred to W (FP32 flight only).
-template <typename F, bool kDefer>
+template <typename F, int kDefer>
 __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View<F>& V,
-                                                 const Sink& sink, Lane<F>& L, Walk<F>& W,
+                                                 const Sink& sink, Lane<F>& L, WalkOf<F, kDefer>& W,
                                                  bool& walking) {
     walking = false;
     const DevSdom<F>& S = V.sdoms[L.sdom];
@@ -1021,14 +1128,14 @@ __device__ __forceinline__ bool emit_now(bool need, bool busy) {
 // next iteration's phase).  With no step to take it runs every walk to its
 // end, so the wavefront's loop always progresses (lanes waiting to emit do
 // not count: emission waits for the walkers).
-template <typename F>
-__device__ __forceinline__ void walk_phase(Walk<F>& W, bool& walking, bool has_work, const Sink& sink) {
+template <int NA>
+__device__ __forceinline__ void walk_phase(WalkN<NA>& W, bool& walking, bool has_work, const Sink& sink) {
     for (;;) {
         const uint64_t m = __ballot(walking);
         if (m == 0ull) break;
         // lanes with a step to take once their walk is over
-        if (__popcll(m) < kWalkQuorum && __ballot(!walking && has_work) != 0ull) break;
-        if (walking && walk_next<F>(W, sink)) walking = false;
+        if (popc(m) < kWalkQuorum && __ballot(!walking && has_work) != 0ull) break;
+        if (walking && walk_next(W, sink)) walking = false;
     }
 }
 
@@ -1074,7 +1181,7 @@ __device__ __forceinline__ int64_t claim(bool need, int lane, uint64_t lt_mask, 
 
 // Single solve: persistent grid, every lane refilled from the solve's global
 // bundle counter until the range [first, first + count) is exhausted.
-template <typename F, bool kLdsTables, bool kLdsTally, bool kDefer>
+template <typename F, bool kLdsTables, bool kLdsTally, int kDefer>
 __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int tid = threadIdx.x;
@@ -1087,7 +1194,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
     int64_t pool_base = 0, pool_lim = 0;
     bool active = false, walking = false;
     Lane<F> L;
-    Walk<F> W;
+    WalkOf<F, kDefer> W;
     LaneStats st;
     if constexpr (sizeof(F) == 4) {
         // FP32: each lane also holds its next bundle, emitted ahead of time
@@ -1138,7 +1245,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
                     active = false;
                 }
             }
-            if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
+            if constexpr (kDefer != 0) walk_phase(W, walking, active, sink);
         }
     } else {
         bool finished = false;
@@ -1163,7 +1270,7 @@ __global__ void __launch_bounds__(1024) trajectory_kernel(KParams<F> K) {
                     active = false;
                 }
             }
-            if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
+            if constexpr (kDefer != 0) walk_phase(W, walking, active, sink);
         }
     }
     st.publish(K.counters, lane);
@@ -1204,7 +1311,7 @@ __global__ void __launch_bounds__(64) trace_from_kernel(KParams<F> K, TraceStart
     L.id = K.trace_id;
     Walk<F> W;
     bool walking = false;
-    while (scatter_step<F, false>(K, V, sink, L, W, walking) == 0xFFFFFFFFu) {
+    while (scatter_step<F, 0>(K, V, sink, L, W, walking) == 0xFFFFFFFFu) {
     }
 }
 
@@ -1218,7 +1325,7 @@ __global__ void __launch_bounds__(64) trace_from_kernel(KParams<F> K, TraceStart
 // every workgroup, and a visit leaves the solve with no unclaimed bundles,
 // so all bundles are processed; a counter read that is stale only makes a
 // visit find nothing to do.
-template <typename F, bool kLdsTables, bool kLdsTally, bool kDefer>
+template <typename F, bool kLdsTables, bool kLdsTally, int kDefer>
 __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restrict__ solves, int nsolves) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_go, s_si;
@@ -1249,7 +1356,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
         int64_t pool_base = 0, pool_lim = 0;
         bool active = false, finished = false, walking = false;
         Lane<F> L;
-        Walk<F> W;
+        WalkOf<F, kDefer> W;
         LaneStats st;
         for (;;) {
             const bool need = !active && !finished && !walking;
@@ -1272,7 +1379,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
                     active = false;
                 }
             }
-            if constexpr (kDefer) walk_phase<F>(W, walking, active, sink);
+            if constexpr (kDefer != 0) walk_phase(W, walking, active, sink);
         }
         st.publish(K.counters, lane);
         __syncthreads();
@@ -1284,25 +1391,27 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
 // ------------------------------------------------------------------ host
 enum class KernelKind { Trajectory, Batch };
 
-template <typename F, bool kTables, bool kTally, bool kDefer>
+template <typename F, bool kTables, bool kTally, int kDefer>
 const void* kernel_of(KernelKind kind) {
     return kind == KernelKind::Trajectory
                ? reinterpret_cast<const void*>(trajectory_kernel<F, kTables, kTally, kDefer>)
                : reinterpret_cast<const void*>(batch_kernel<F, kTables, kTally, kDefer>);
 }
 
+template <typename F, int kDefer>
+const void* kernel_by_plan(KernelKind kind, bool tables, bool tally) {
+    return tables ? (tally ? kernel_of<F, true, true, kDefer>(kind) : kernel_of<F, true, false, kDefer>(kind))
+                  : (tally ? kernel_of<F, false, true, kDefer>(kind) : kernel_of<F, false, false, kDefer>(kind));
+}
+
 // The kernel variant for an LDS plan; deferred walks only for FP32 flight on
-// scenes with 2-D / 3-D tallies.
+// scenes with 2-D / 3-D tallies, over two axes when no grid is 3-D.
 template <typename F>
-const void* pick_kernel(KernelKind kind, bool tables, bool tally, bool grid) {
+const void* pick_kernel(KernelKind kind, bool tables, bool tally, bool grid, bool grid_2d) {
     if constexpr (sizeof(F) == 4) {
-        if (grid) {
-            return tables ? (tally ? kernel_of<F, true, true, true>(kind) : kernel_of<F, true, false, true>(kind))
-                          : (tally ? kernel_of<F, false, true, true>(kind) : kernel_of<F, false, false, true>(kind));
-        }
+        if (grid) return grid_2d ? kernel_by_plan<F, 2>(kind, tables, tally) : kernel_by_plan<F, 3>(kind, tables, tally);
     }
-    return tables ? (tally ? kernel_of<F, true, true, false>(kind) : kernel_of<F, true, false, false>(kind))
-                  : (tally ? kernel_of<F, false, true, false>(kind) : kernel_of<F, false, false, false>(kind));
+    return kernel_by_plan<F, 0>(kind, tables, tally);
 }
 
 thread_local std::string t_err;
@@ -1369,6 +1478,8 @@ void convert_sdom(const DevSdom<double>& s, DevSdom<F>& d, double eps_scale) {
     d.eps = F(s.eps * eps_scale);
     d.accum = s.accum; d.pbeg = s.pbeg; d.pend = s.pend; d.col_off = s.col_off; d.pore = s.pore;
     d.box = s.box;
+    d.gax[0] = s.gax[0];
+    d.gax[1] = s.gax[1];
     for (int i = 0; i < 3; ++i) {
         d.bmin[i] = F(s.box ? s.bmin[i] : 0.0);
         d.bmax[i] = F(s.box ? s.bmax[i] : 0.0);
@@ -1558,7 +1669,7 @@ int launch_solve(mcg_scene* sc, const HostProblem& p, KParams<F>& K, hipStream_t
     const size_t lds = lp.bytes;
     if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     K.lds_tally = tally ? 1 : 0;
-    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally, h.has_grid);
+    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally, h.has_grid, h.grid_2d);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
     // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
@@ -1825,6 +1936,7 @@ struct BatchPlan {
     size_t lds = 0;
     bool tables = true, tally = true;
     bool grid_tally = false;   // some solve tallies on a 2-D / 3-D grid
+    bool grid_2d = true;       // no solve's grid is 3-D
     int block = 256, grid = 0, chunk = kChunk;
 };
 
@@ -1843,6 +1955,7 @@ int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
         tab = std::max(tab, lds_bytes<F>(h, int(P[i].rows), true, false) - g);
         tally_bytes = std::max(tally_bytes, lds_bytes<F>(h, int(P[i].rows), false, true) - g);
         grid = grid || h.has_grid;
+        B.grid_2d = B.grid_2d && h.grid_2d;
     }
     const LdsPlan lp = plan_lds(geo, tab, tally_bytes, grid, !(tune && tune->lds_tables < 0), s0->lds_limit);
     B.grid_tally = grid;
@@ -1888,7 +2001,7 @@ int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
 // Enqueue one batch launch on `stream`.
 template <typename F>
 int launch_batch(const BatchPlan& B, const KParams<F>* kp, int nsolves, hipStream_t stream) {
-    const void* fn = pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally);
+    const void* fn = pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally, B.grid_2d);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
     void* args[] = {&kp, &nsolves};
     HIP_TRY(hipLaunchKernel(fn, dim3(std::max(1, B.grid)), dim3(B.block), args, B.lds, stream));
@@ -2071,7 +2184,7 @@ struct GraphBatch : GraphBatchBase {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         // the kernel's LDS limit is an attribute of the function, set before capture
-        HIP_TRY(hipFuncSetAttribute(pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally),
+        HIP_TRY(hipFuncSetAttribute(pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally, B.grid_2d),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         const size_t total = direct.n + diff.n + ctr.n;

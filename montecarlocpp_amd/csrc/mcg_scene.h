@@ -68,6 +68,7 @@ struct DevSdom {
     int32_t stride[3];
     int32_t pore;     // index of the cylindrical pore or -1
     int32_t box;      // 1: six axis-aligned faces (the FP32 flight's box advect)
+    int32_t gax[2];   // 2-D grid (accum 3): its two divided axes
     F bmin[3], bmax[3];        // box: the faces' positions on each axis
     F kmin[3], kmax[3];        // box: positions beyond which a bundle is lost (eps outside)
     int32_t fmin[3], fmax[3];  // box: plane offsets (from pbeg) of the low / high face
@@ -123,6 +124,7 @@ struct HostScene {
     int64_t cols = 0;
     bool has_1d = false;     // some subdomain tallies along one axis (range adds)
     bool has_grid = false;   // some subdomain tallies on a 2-D / 3-D grid
+    bool grid_2d = false;    // no grid is 3-D (with has_grid: every grid is 2-D, accum 3)
 };
 
 struct HostProblem {
