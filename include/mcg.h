@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MCG_ABI_VERSION 1
+#define MCG_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- status */
 enum mcg_status {
@@ -307,12 +307,27 @@ int mcg_trace_from(mcg_scene* scene, const mcg_problem_spec* prob, int64_t prop_
 /* --------------------------------------------------------------- batching
  * Many independent solves in one persistent launch.  Each solve has its
  * own scene, problem and output; results equal those of mcg_solve. */
+enum mcg_engine {
+    MCG_ENGINE_HISTORY = 0,    /* persistent kernel, a bundle per lane for its
+                                  whole life, state in registers (default)    */
+    MCG_ENGINE_EVENT = 1       /* structure-of-arrays state in HBM, one step
+                                  per bundle per round, bundles regrouped by
+                                  next event and cell kind between rounds     */
+};
+
 struct mcg_tune {
     int32_t block;             /* threads per block: a multiple of 64 up to 1024 (0 = auto) */
-    int32_t per_thread;        /* phonons carried per thread: 1, 2, 4, 8        */
-    int32_t chunk;             /* bundles claimed per wavefront grab            */
+    int32_t per_thread;        /* event engine: bundles per thread per round
+                                  launch, 1, 2, 4 or 8 (0 = 1); the history
+                                  engine carries one bundle per lane          */
+    int32_t chunk;             /* history: bundle ids per wavefront grab         */
     int32_t blocks_per_cu;     /* occupancy target (0 = auto)                   */
     int32_t lds_tables;        /* sampling tables in LDS: 1 yes, -1 no, 0 auto  */
+    int32_t engine;            /* mcg_engine (single solves, mcg_solve)         */
+    int32_t compact_every;     /* event engine: rounds between regroupings, the
+                                  others keep bundles in place (0 = 1)         */
+    int32_t reserved;
+    int64_t live;              /* event engine: bundles in flight (0 = auto)    */
 };
 
 int mcg_batch_solve(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,

@@ -102,9 +102,14 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+# engines of a single solve
+ENGINE_HISTORY, ENGINE_EVENT = range(2)
+
+
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("per_thread", C.c_int32), ("chunk", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("lds_tables", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("lds_tables", C.c_int32), ("engine", C.c_int32),
+                ("compact_every", C.c_int32), ("reserved", C.c_int32), ("live", C.c_int64)]
 
 
 def dptr(a):

@@ -825,14 +825,16 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
     }
 }
 
-// One scatter-step of the bundle in L (problem.cpp:401-435): advect, tally,
-// boundary or intrinsic scatter, termination test.  Returns the bundle's fate
-// when it ends, 0xFFFFFFFF while it lives.  `walking` is set when the step's
-// 2-D / 3-D tally walk was deferred to W (FP32 flight only).
+// The flight half of a scatter-step (problem.cpp:401-416): advect to the
+// nearest face, pore wall or intrinsic event, the kill test, and the tally of
+// the flight.  Returns the event (a plane index, -1 intrinsic, -3 the pore)
+// and sets fate to MCG_ESCAPED when the bundle is lost; dmin and apos receive
+// the distance and the position after the flight.  `walking` is set when the
+// flight's 2-D / 3-D tally walk was deferred to W (FP32 flight only).
 template <typename F, int kDefer>
-__device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View<F>& V,
-                                                 const Sink& sink, Lane<F>& L, WalkOf<F, kDefer>& W,
-                                                 bool& walking) {
+__device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V, const Sink& sink,
+                                           Lane<F>& L, WalkOf<F, kDefer>& W, bool& walking, uint32_t& fate,
+                                           F& dmin_out, dev::V3<F>& apos) {
     walking = false;
     const DevSdom<F>& S = V.sdoms[L.sdom];
     const F vel = V.vel[L.mode];
@@ -951,8 +953,8 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     const dev::V3<F> pre = L.pos;
     L.pos = dev::V3<F>{pre.x + dmin * L.dir.x, pre.y + dmin * L.dir.y, pre.z + dmin * L.dir.z};
     L.nstep++;
-    const dev::V3<F> apos = L.pos;
-    uint32_t fate = 0xFFFFFFFFu;
+    apos = L.pos;
+    dmin_out = dmin;
     // kill test (subdomain.cpp:186-190): outside any face by more than eps
     bool outside;
     if constexpr (sizeof(F) == 4) {
@@ -1000,8 +1002,19 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             r0 = tk == MCG_CUMTEMP ? bin : 3 * bin;
         }
         walking = accumulate<F, kDefer>(S, sink, pre, L.pos, r0, nv, amt, W);
+    }
+    return hit;
+}
 
-        // scatter: boundary (boundary.cpp:283-522) or intrinsic (material.cpp:226-231)
+// The collision half of a scatter-step (problem.cpp:418-433): the boundary
+// scatter (boundary.cpp:283-522) or intrinsic scatter (material.cpp:226-231)
+// of the event `hit` found by the flight, then the termination test.  The
+// bundle's subdomain is still the flight's.  Sets fate when the bundle ends.
+template <typename F>
+__device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>& V, int hit, Lane<F>& L,
+                                             uint32_t& fate) {
+    const DevSdom<F>& S = V.sdoms[L.sdom];
+    {
         // the event's Philox blocks serve every branch below; they are drawn
         // once, outside the branches, so that a wavefront whose lanes take
         // different branches evaluates them once (FP32: block 0 alone)
@@ -1084,6 +1097,21 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
             else if (L.nstep >= K.maxloop) fate = MCG_MAXLOOP;
         }
     }
+}
+
+// One scatter-step of the bundle in L (problem.cpp:401-435): the flight and
+// the collision.  Returns the bundle's fate when it ends, 0xFFFFFFFF while it
+// lives.  `walking` is set when the step's 2-D / 3-D tally walk was deferred
+// to W (FP32 flight only).
+template <typename F, int kDefer>
+__device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View<F>& V,
+                                                 const Sink& sink, Lane<F>& L, WalkOf<F, kDefer>& W,
+                                                 bool& walking) {
+    uint32_t fate = 0xFFFFFFFFu;
+    F dmin;
+    dev::V3<F> apos;
+    const int hit = flight_step<F, kDefer>(K, V, sink, L, W, walking, fate, dmin, apos);
+    if (fate == 0xFFFFFFFFu) collide_step<F>(K, V, hit, L, fate);
     if (L.id == K.trace_id && int64_t(L.nstep) <= K.trace_cap) {
         double* r = K.trace + (int64_t(L.nstep) - 1) * MCG_TRACE_WIDTH;
         r[0] = double(L.nstep);
@@ -1388,6 +1416,296 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
     }
 }
 
+// ================================================================ event engine
+// The event-based variant of the solve: bundle state in HBM as structure of
+// arrays, advanced one scatter-step per bundle per round.  A round performs
+// each live bundle's pending collision (the event its last flight found),
+// then its next flight and tally, and appends the survivor to the next
+// round's list under the key (class of the event just found: intrinsic,
+// planar face or pore wall; box cell or other), so that a wavefront of the
+// next round runs one collision branch on one advect path.  Isothermal walls
+// absorb at once and lost bundles leave at once.  Emission refills the list to
+// a target number of live bundles; when no bundle is left to emit and the
+// list is short, a drain launch finishes the rest history-style.  Every
+// bundle takes exactly the flights and collisions of the history kernel, with
+// the same Philox events, so results equal it bundle for bundle (tallies up
+// to FP64 summation order).
+constexpr int kEvGroups = 6;
+constexpr int32_t kEvHole = -1000;   // pending-event mark of a slot left empty in place
+
+template <typename F>
+struct EvState {
+    F *px, *py, *pz, *ux, *uy, *uz, *sn;
+    uint32_t* mode;
+    int32_t* sds;        // subdomain << 1 | (sign < 0)
+    uint32_t *nscat, *nbdry, *nstep;
+    int64_t* id;
+    int32_t* hit;        // pending event (plane, -1 intrinsic, -3 pore), kEvHole for an empty slot
+};
+
+template <typename F>
+struct EvParams {
+    KParams<F> K;
+    EvState<F> in, out;                 // kEvGroups regions of `cap` slots each
+    unsigned long long* in_count;       // entries per group of the input list
+    unsigned long long* out_count;      // entries per group of the output list
+    unsigned long long* emit;           // [0] first id offset, [1] bundles to emit this round
+    int64_t cap;
+    int64_t live_target;
+    int32_t regroup;                    // 1: survivors appended to `out` by group; 0: kept in place
+};
+
+template <typename F>
+__device__ __forceinline__ void ev_load(const EvState<F>& S, int64_t s, Lane<F>& L, int& hit) {
+    L.pos = dev::V3<F>{S.px[s], S.py[s], S.pz[s]};
+    L.dir = dev::V3<F>{S.ux[s], S.uy[s], S.uz[s]};
+    L.scat_next = S.sn[s];
+    L.mode = S.mode[s];
+    const int32_t sd = S.sds[s];
+    L.sdom = sd >> 1;
+    L.sign = (sd & 1) ? -1 : 1;
+    L.nscat = S.nscat[s];
+    L.nbdry = S.nbdry[s];
+    L.nstep = S.nstep[s];
+    L.id = S.id[s];
+    hit = S.hit[s];
+}
+
+template <typename F>
+__device__ __forceinline__ void ev_store(const EvState<F>& S, int64_t s, const Lane<F>& L, int hit) {
+    S.px[s] = L.pos.x;
+    S.py[s] = L.pos.y;
+    S.pz[s] = L.pos.z;
+    S.ux[s] = L.dir.x;
+    S.uy[s] = L.dir.y;
+    S.uz[s] = L.dir.z;
+    S.sn[s] = L.scat_next;
+    S.mode[s] = L.mode;
+    S.sds[s] = (L.sdom << 1) | (L.sign < 0 ? 1 : 0);
+    S.nscat[s] = L.nscat;
+    S.nbdry[s] = L.nbdry;
+    S.nstep[s] = L.nstep;
+    S.id[s] = L.id;
+    S.hit[s] = hit;
+}
+
+// Slot of entry e of a list whose groups hold count[0..kEvGroups) entries
+// (group g occupies slots [g cap, g cap + count[g]))
+__device__ __forceinline__ int64_t ev_slot(const unsigned long long (&cnt)[kEvGroups], int64_t e, int64_t cap) {
+    int64_t acc = 0;
+#pragma unroll
+    for (int g = 0; g < kEvGroups; ++g) {
+        const int64_t c = int64_t(cnt[g]);
+        if (e < acc + c) return int64_t(g) * cap + (e - acc);
+        acc += c;
+    }
+    return -1;
+}
+
+// Append the lanes with `want` to group `grp` of a list: one atomic per group
+// and wavefront; returns the lane's slot (or -1).  All lanes call.
+__device__ __forceinline__ int64_t ev_append(unsigned long long* count, int64_t cap, int grp, bool want, int lane,
+                                             uint64_t lt_mask) {
+    int64_t slot = -1;
+#pragma unroll
+    for (int g = 0; g < kEvGroups; ++g) {
+        const uint64_t m = __ballot(want && grp == g);
+        if (m == 0ull) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(count + g, (unsigned long long)popc(m));
+        base = __shfl(base, leader);
+        if (want && grp == g) slot = int64_t(g) * cap + int64_t(base) + popc(m & lt_mask);
+    }
+    return slot;
+}
+
+// The next flight of a bundle whose collision left it alive: advect, kill
+// test, tally (walks inline), and an isothermal wall's absorption at once.
+template <typename F>
+__device__ __forceinline__ int ev_flight(const KParams<F>& K, const View<F>& V, const Sink& sink, Lane<F>& L,
+                                         uint32_t& fate) {
+    Walk<F> W;
+    bool walking = false;
+    F dmin;
+    dev::V3<F> apos;
+    const int hit = flight_step<F, 0>(K, V, sink, L, W, walking, fate, dmin, apos);
+    if (fate == 0xFFFFFFFFu && hit >= 0 && V.planes[hit].kind == MCG_ISOT) collide_step<F>(K, V, hit, L, fate);
+    return hit;
+}
+
+template <typename F>
+__device__ __forceinline__ int ev_group(const View<F>& V, const Lane<F>& L, int hit) {
+    const int cls = hit == -1 ? 0 : (hit == -3 ? 2 : 1);
+    return 2 * cls + (V.sdoms[L.sdom].box ? 0 : 1);
+}
+
+template <typename F>
+__device__ __forceinline__ void ev_finish(const KParams<F>& K, const Lane<F>& L, uint32_t fate, LaneStats& st) {
+    st.add(fate, L.nstep);
+    if (K.c_nscat) {
+        const int64_t k = L.id - K.first;
+        K.c_nscat[k] = L.nscat;
+        K.c_nbdry[k] = L.nbdry;
+        K.c_nstep[k] = L.nstep;
+        K.c_fate[k] = fate;
+    }
+}
+
+// One round over the input list.
+template <typename F, bool kLdsTally>
+__global__ void __launch_bounds__(1024) ev_round_kernel(EvParams<F> E) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const KParams<F>& K = E.K;
+    const int tid = threadIdx.x;
+    const View<F> V = stage<F, false, kLdsTally>(K, lds, tid, blockDim.x);
+    const Sink sink{V.direct, V.diff, K.cols};
+    __syncthreads();
+    const int lane = tid & (kWave - 1);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    unsigned long long cnt[kEvGroups];
+    int64_t total = 0;
+#pragma unroll
+    for (int g = 0; g < kEvGroups; ++g) {
+        cnt[g] = E.in_count[g];
+        total += int64_t(cnt[g]);
+    }
+    LaneStats st;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < total; base += stride) {
+        const int64_t e = base + tid;
+        const int64_t s = e < total ? ev_slot(cnt, e, E.cap) : -1;
+        Lane<F> L;
+        int hit = kEvHole;
+        if (s >= 0) ev_load(E.in, s, L, hit);
+        bool live = hit != kEvHole;
+        int grp = 0;
+        if (live) {
+            uint32_t fate = 0xFFFFFFFFu;
+            collide_step<F>(K, V, hit, L, fate);
+            if (fate == 0xFFFFFFFFu) hit = ev_flight<F>(K, V, sink, L, fate);
+            if (fate != 0xFFFFFFFFu) {
+                ev_finish(K, L, fate, st);
+                live = false;
+            } else {
+                grp = ev_group(V, L, hit);
+            }
+        }
+        if (E.regroup) {
+            const int64_t o = ev_append(E.out_count, E.cap, grp, live, lane, lt_mask);
+            if (live) ev_store(E.out, o, L, hit);
+        } else if (s >= 0) {
+            // kept in place: a bundle that ended leaves a hole
+            if (live) ev_store(E.in, s, L, hit);
+            else if (hit != kEvHole) E.in.hit[s] = kEvHole;
+        }
+    }
+    st.publish(K.counters, lane);
+    if (kLdsTally) {
+        __syncthreads();
+        flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, K.lds_diff != 0, tid, blockDim.x);
+    }
+}
+
+// How many bundles this round emits (up to the live target, within the
+// bundles left), and the clearing of the counters the next round appends to.
+__global__ void ev_prep_kernel(unsigned long long* out_count, unsigned long long* next_count,
+                               unsigned long long* next_id, unsigned long long* emit, int64_t count,
+                               int64_t live_target) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t live = 0;
+    for (int g = 0; g < kEvGroups; ++g) live += int64_t(out_count[g]);
+    const int64_t left = count - int64_t(*next_id);
+    const int64_t want = live_target > live ? live_target - live : 0;
+    const int64_t n = want < left ? want : left;
+    emit[0] = *next_id;
+    emit[1] = (unsigned long long)(n > 0 ? n : 0);
+    *next_id += emit[1];
+    for (int g = 0; g < kEvGroups; ++g) next_count[g] = 0ull;
+}
+
+// Emission of this round's new bundles and their first flight; appended to
+// the output list.
+template <typename F, bool kLdsTally>
+__global__ void __launch_bounds__(1024) ev_emit_kernel(EvParams<F> E) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const KParams<F>& K = E.K;
+    const int tid = threadIdx.x;
+    const View<F> V = stage<F, false, kLdsTally>(K, lds, tid, blockDim.x);
+    const Sink sink{V.direct, V.diff, K.cols};
+    __syncthreads();
+    const int lane = tid & (kWave - 1);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    const int64_t first = int64_t(E.emit[0]), n = int64_t(E.emit[1]);
+    LaneStats st;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < n; base += stride) {
+        const int64_t t = base + tid;
+        Lane<F> L;
+        bool live = false;
+        int hit = 0, grp = 0;
+        if (t < n) {
+            emit<F>(K, K.flux_alias, V.mfp, K.first + first + t, L);
+            uint32_t fate = 0xFFFFFFFFu;
+            hit = ev_flight<F>(K, V, sink, L, fate);
+            if (fate != 0xFFFFFFFFu) {
+                ev_finish(K, L, fate, st);
+            } else {
+                live = true;
+                grp = ev_group(V, L, hit);
+            }
+        }
+        const int64_t o = ev_append(E.out_count, E.cap, grp, live, lane, lt_mask);
+        if (live) ev_store(E.out, o, L, hit);
+    }
+    st.publish(K.counters, lane);
+    if (kLdsTally) {
+        __syncthreads();
+        flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, K.lds_diff != 0, tid, blockDim.x);
+    }
+}
+
+// The tail: every bundle left in the list runs to its end, history-style.
+template <typename F, bool kLdsTally>
+__global__ void __launch_bounds__(1024) ev_drain_kernel(EvParams<F> E) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const KParams<F>& K = E.K;
+    const int tid = threadIdx.x;
+    const View<F> V = stage<F, false, kLdsTally>(K, lds, tid, blockDim.x);
+    const Sink sink{V.direct, V.diff, K.cols};
+    __syncthreads();
+    const int lane = tid & (kWave - 1);
+    unsigned long long cnt[kEvGroups];
+    int64_t total = 0;
+#pragma unroll
+    for (int g = 0; g < kEvGroups; ++g) {
+        cnt[g] = E.in_count[g];
+        total += int64_t(cnt[g]);
+    }
+    LaneStats st;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + tid; e < total; e += stride) {
+        const int64_t s = ev_slot(cnt, e, E.cap);
+        Lane<F> L;
+        int hit = kEvHole;
+        ev_load(E.in, s, L, hit);
+        if (hit == kEvHole) continue;
+        uint32_t fate = 0xFFFFFFFFu;
+        for (;;) {
+            collide_step<F>(K, V, hit, L, fate);
+            if (fate != 0xFFFFFFFFu) break;
+            hit = ev_flight<F>(K, V, sink, L, fate);
+            if (fate != 0xFFFFFFFFu) break;
+        }
+        ev_finish(K, L, fate, st);
+    }
+    st.publish(K.counters, lane);
+    if (kLdsTally) {
+        __syncthreads();
+        flush_tally(V.direct, V.diff, K.direct, K.diff, K.rows, K.cols, K.lds_diff != 0, tid, blockDim.x);
+    }
+}
+
 // ------------------------------------------------------------------ host
 enum class KernelKind { Trajectory, Batch };
 
@@ -1551,7 +1869,7 @@ struct mcg_scene {
     int device = 0;
     int num_cu = 0;
     size_t lds_limit = 0;
-    mcg_tune tune{0, 0, 0, 0, 0};   // launch shape for this scene's solves (0 = default)
+    mcg_tune tune{};                // launch shape and engine of this scene's solves (0 = default)
     // per-solve device workspace, kept between solves (one thread solves a
     // scene at a time): tallies (direct, difference array, then the
     // counters, read back in one copy into pinned host memory), emitter CDF
@@ -1562,6 +1880,9 @@ struct mcg_scene {
     double* pinned = nullptr;
     size_t pinned_n = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // event engine: two bundle lists (structure of arrays) and their counters
+    mcg::DevBuf<unsigned char> ev_state;
+    mcg::DevBuf<unsigned long long> ev_ctl;
     ~mcg_scene() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1879,6 +2200,237 @@ int run_solve_static(mcg_scene* sc, const HostProblem& p, double* out, mcg_stats
     }
     if (int64_t(hc[C_BUNDLES]) != p.count)
         return set_error(MCG_EDEVICE, "kernel finished with bundles unaccounted for");
+    return MCG_OK;
+}
+
+// ------------------------------------------------------------ event engine host
+// The bundle lists of the event engine: kEvGroups regions of `cap` slots,
+// one array per field, carved out of one allocation.
+template <typename F>
+EvState<F> ev_carve(unsigned char* base, int64_t slots) {
+    EvState<F> S;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += (bytes * size_t(slots) + 255) & ~size_t(255);
+        return p;
+    };
+    S.px = reinterpret_cast<F*>(take(sizeof(F)));
+    S.py = reinterpret_cast<F*>(take(sizeof(F)));
+    S.pz = reinterpret_cast<F*>(take(sizeof(F)));
+    S.ux = reinterpret_cast<F*>(take(sizeof(F)));
+    S.uy = reinterpret_cast<F*>(take(sizeof(F)));
+    S.uz = reinterpret_cast<F*>(take(sizeof(F)));
+    S.sn = reinterpret_cast<F*>(take(sizeof(F)));
+    S.mode = reinterpret_cast<uint32_t*>(take(4));
+    S.sds = reinterpret_cast<int32_t*>(take(4));
+    S.nscat = reinterpret_cast<uint32_t*>(take(4));
+    S.nbdry = reinterpret_cast<uint32_t*>(take(4));
+    S.nstep = reinterpret_cast<uint32_t*>(take(4));
+    S.id = reinterpret_cast<int64_t*>(take(8));
+    S.hit = reinterpret_cast<int32_t*>(take(4));
+    return S;
+}
+
+template <typename F>
+size_t ev_list_bytes(int64_t slots) {
+    size_t total = 0;
+    const size_t sizes[] = {sizeof(F), sizeof(F), sizeof(F), sizeof(F), sizeof(F), sizeof(F), sizeof(F),
+                            4, 4, 4, 4, 4, 8, 4};
+    for (size_t b : sizes) total += (b * size_t(slots) + 255) & ~size_t(255);
+    return total;
+}
+
+template <typename F>
+const void* ev_kernel(int which, bool tally) {
+    switch (which) {
+        case 0: return tally ? reinterpret_cast<const void*>(ev_round_kernel<F, true>)
+                             : reinterpret_cast<const void*>(ev_round_kernel<F, false>);
+        case 1: return tally ? reinterpret_cast<const void*>(ev_emit_kernel<F, true>)
+                             : reinterpret_cast<const void*>(ev_emit_kernel<F, false>);
+        default: return tally ? reinterpret_cast<const void*>(ev_drain_kernel<F, true>)
+                              : reinterpret_cast<const void*>(ev_drain_kernel<F, false>);
+    }
+}
+
+// mcg_solve by the event engine (tune.engine = MCG_ENGINE_EVENT).
+template <typename F>
+int run_solve_event(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* counts, mcg_stats* stats) {
+    const HostScene& h = sc->host;
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(sc->device));
+    const int rows = int(p.rows);
+    const int cols = int(h.cols);
+    if (p.rows > (1 << 20) || h.cols * p.rows > (1ll << 31)) return set_error(MCG_ERANGE, "tally too large");
+    if (p.maxloop <= 0) return set_error(MCG_EINVAL, "maxloop must be positive");
+    const size_t nd = size_t(rows) * cols, nf = size_t(rows) * (cols + 1);
+    const size_t nall = nd + nf + C_COUNT;
+    int rc;
+    if ((rc = sc->ws_tally.reserve(nall))) return rc;
+    if (sc->ws_cdf.p == nullptr || sc->ws_cdf.n < p.emit_cdf.size() || sc->ws_cdf_host != p.emit_cdf) {
+        if ((rc = sc->ws_cdf.reserve(p.emit_cdf.size()))) return rc;
+        HIP_TRY(hipMemcpy(sc->ws_cdf.p, p.emit_cdf.data(), p.emit_cdf.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+        sc->ws_cdf_host = p.emit_cdf;
+    }
+    if (sc->pinned_n < nall) {
+        if (sc->pinned) HIP_TRY(hipHostFree(sc->pinned));
+        sc->pinned = nullptr;
+        sc->pinned_n = 0;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&sc->pinned), nall * sizeof(double), hipHostMallocDefault));
+        sc->pinned_n = nall;
+    }
+    // bundles in flight and the lists holding them
+    const int64_t live = std::max<int64_t>(1, std::min<int64_t>(p.count, sc->tune.live > 0 ? sc->tune.live
+                                                                                          : (int64_t(1) << 21)));
+    const int64_t cap = live;
+    const int64_t slots = int64_t(kEvGroups) * cap;
+    const size_t list_bytes = ev_list_bytes<F>(slots);
+    if ((rc = sc->ev_state.reserve(2 * list_bytes))) return rc;
+    // counters: list A (kEvGroups), list B (kEvGroups), emit (2), next id (1)
+    if ((rc = sc->ev_ctl.reserve(2 * kEvGroups + 3))) return rc;
+    unsigned long long* const cA = sc->ev_ctl.p;
+    unsigned long long* const cB = cA + kEvGroups;
+    unsigned long long* const emitw = cB + kEvGroups;
+    unsigned long long* const next_id = emitw + 2;
+    DevBuf<uint32_t> cnt;
+    const bool want_counts = counts && counts->nscat && counts->nbdry && counts->nstep && counts->fate;
+    if (want_counts && (rc = cnt.alloc(size_t(4) * p.count))) return rc;
+
+    double* const direct = sc->ws_tally.p;
+    double* const diff = direct + nd;
+    unsigned long long* const ctr = reinterpret_cast<unsigned long long*>(direct + nd + nf);
+    hipLaunchKernelGGL(zero_kernel, dim3(int(std::min<size_t>(1024, (nd + nf + 255) / 256 + 1))), dim3(256), 0, 0,
+                       direct, nd + nf, static_cast<double*>(nullptr), size_t(0), ctr, size_t(C_COUNT));
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, 0, static_cast<double*>(nullptr), size_t(0),
+                       static_cast<double*>(nullptr), size_t(0), sc->ev_ctl.p, size_t(2 * kEvGroups + 3));
+    HIP_TRY(hipGetLastError());
+
+    EvParams<F> E{};
+    fill_params<F>(sc, p, E.K);
+    E.K.emit_cdf = sc->ws_cdf.p;
+    E.K.direct = direct;
+    E.K.diff = diff;
+    E.K.counters = ctr;
+    E.K.next = ctr + C_NEXT;
+    if (want_counts) {
+        E.K.c_nscat = cnt.p;
+        E.K.c_nbdry = cnt.p + p.count;
+        E.K.c_nstep = cnt.p + 2 * p.count;
+        E.K.c_fate = cnt.p + 3 * p.count;
+    }
+    E.cap = cap;
+    E.live_target = live;
+    EvState<F> SA = ev_carve<F>(sc->ev_state.p, slots), SB = ev_carve<F>(sc->ev_state.p + list_bytes, slots);
+    // LDS: geometry, and the tally when it fits (the tables stay in global
+    // memory: every launch stages what it uses)
+    const size_t geo = lds_bytes<F>(h, rows, false, false);
+    const LdsPlan lp = plan_lds(geo, 0, lds_bytes<F>(h, rows, false, true) - geo, h.has_grid, false, sc->lds_limit);
+    if (lp.bytes > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
+    E.K.lds_tally = lp.tally ? 1 : 0;
+    const int block = sc->tune.block > 0 ? sc->tune.block : 256;
+    int per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lp.bytes, 1)));
+    per_cu = std::min(per_cu, std::max(1, 2048 / block));
+    if (sc->tune.blocks_per_cu > 0) per_cu = std::min(per_cu, int(sc->tune.blocks_per_cu));
+    const int64_t max_grid = int64_t(sc->num_cu) * per_cu;
+    const int per_thread = sc->tune.per_thread > 0 ? sc->tune.per_thread : 1;
+    const int compact_every = sc->tune.compact_every > 0 ? sc->tune.compact_every : 1;
+    for (int w = 0; w < 3; ++w)
+        HIP_TRY(hipFuncSetAttribute(ev_kernel<F>(w, lp.tally), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lp.bytes)));
+    auto grid_for = [&](int64_t n) {
+        const int64_t want = (n + int64_t(block) * per_thread - 1) / (int64_t(block) * per_thread);
+        return int(std::max<int64_t>(1, std::min<int64_t>(want, max_grid)));
+    };
+    auto launch = [&](int which, const EvParams<F>& P, int grid) -> int {
+        EvParams<F> P2 = P;
+        void* args[] = {&P2};
+        HIP_TRY(hipLaunchKernel(ev_kernel<F>(which, lp.tally), dim3(grid), dim3(block), args, lp.bytes, 0));
+        HIP_TRY(hipGetLastError());
+        return MCG_OK;
+    };
+    if (!sc->ev0) HIP_TRY(hipEventCreate(&sc->ev0));
+    if (!sc->ev1) HIP_TRY(hipEventCreate(&sc->ev1));
+    HIP_TRY(hipEventRecord(sc->ev0, 0));
+    // A holds the current list; the first emission fills it
+    E.in = SB;
+    E.out = SA;
+    E.in_count = cB;
+    E.out_count = cA;
+    E.emit = emitw;
+    hipLaunchKernelGGL(ev_prep_kernel, dim3(1), dim3(64), 0, 0, cA, cB, next_id, emitw, p.count, live);
+    if ((rc = launch(1, E, grid_for(live)))) return rc;
+    bool a_current = true;
+    int64_t rounds = 0;
+    const int64_t drain_at = std::max<int64_t>(int64_t(sc->num_cu) * 256, live / 16);
+    for (;;) {
+        // a batch of rounds, then one look at the lists
+        for (int r = 0; r < 8; ++r, ++rounds) {
+            EvParams<F> R = E;
+            R.in = a_current ? SA : SB;
+            R.out = a_current ? SB : SA;
+            R.in_count = a_current ? cA : cB;
+            R.out_count = a_current ? cB : cA;
+            if ((rounds + 1) % compact_every != 0) {
+                // in place: holes stay until the next regrouping
+                R.regroup = 0;
+                if ((rc = launch(0, R, grid_for(live)))) return rc;
+                continue;
+            }
+            R.regroup = 1;
+            if ((rc = launch(0, R, grid_for(live)))) return rc;
+            // emission up to the live target into the new list; the old
+            // list's counters are cleared for the round after
+            hipLaunchKernelGGL(ev_prep_kernel, dim3(1), dim3(64), 0, 0, R.out_count, R.in_count, next_id, emitw,
+                               p.count, live);
+            if ((rc = launch(1, R, grid_for(live)))) return rc;
+            a_current = !a_current;
+        }
+        unsigned long long hc[2 * kEvGroups + 3];
+        HIP_TRY(hipMemcpy(hc, sc->ev_ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
+        const unsigned long long* cur = a_current ? hc : hc + kEvGroups;
+        int64_t in_list = 0;
+        for (int g = 0; g < kEvGroups; ++g) in_list += int64_t(cur[g]);
+        const bool emitted_all = int64_t(hc[2 * kEvGroups + 2]) >= p.count;
+        if (emitted_all && in_list <= drain_at) {
+            if (in_list > 0) {
+                EvParams<F> R = E;
+                R.in = a_current ? SA : SB;
+                R.in_count = a_current ? cA : cB;
+                if ((rc = launch(2, R, grid_for(in_list)))) return rc;
+            }
+            break;
+        }
+        // every round takes each live bundle one step, so a solve ends within
+        // maxloop rounds of its last emission; more means a fault
+        if (rounds > std::min<int64_t>(p.maxloop, int64_t(1) << 20) + (p.count / live + 1) * 8 + 64)
+            return set_error(MCG_EDEVICE, "event engine did not finish within maxloop rounds");
+    }
+    HIP_TRY(hipEventRecord(sc->ev1, 0));
+    HIP_TRY(hipMemcpyAsync(sc->pinned, direct, nall * sizeof(double), hipMemcpyDeviceToHost, 0));
+    HIP_TRY(hipStreamSynchronize(0));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, sc->ev0, sc->ev1));
+    unsigned long long hc[C_COUNT];
+    std::memcpy(hc, sc->pinned + nd + nf, sizeof(hc));
+    if (want_counts) {
+        HIP_TRY(hipMemcpy(counts->nscat, cnt.p, p.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(counts->nbdry, cnt.p + p.count, p.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(counts->nstep, cnt.p + 2 * p.count, p.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(counts->fate, cnt.p + 3 * p.count, p.count * 4, hipMemcpyDeviceToHost));
+    }
+    finalise_field(h, p, sc->pinned, sc->pinned + nd, out);
+    if (stats) {
+        stats->bundles = int64_t(hc[C_BUNDLES]);
+        stats->steps = int64_t(hc[C_STEPS]);
+        stats->escapes = int64_t(hc[C_ESC]);
+        stats->absorbed = int64_t(hc[C_ABS]);
+        stats->truncated = int64_t(hc[C_TRUNC]);
+        stats->kernel_ms = ms;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (int64_t(hc[C_BUNDLES]) != p.count)
+        return set_error(MCG_EDEVICE, "event engine finished with bundles unaccounted for");
     return MCG_OK;
 }
 
@@ -2297,7 +2849,7 @@ struct StepSlot {
 struct mcg_server {
     size_t budget = 0, used = 0;
     int64_t step_bundles = 0;
-    mcg_tune tune{0, 0, 0, 0, 0};
+    mcg_tune tune{};
     int64_t next_id = 1;
     std::deque<mcg::Request*> queue;
     std::vector<mcg::Request*> active;
@@ -2591,11 +3143,15 @@ int mcg_scene_plan(const mcg_scene* scene, const mcg_problem_spec* prob, mcg_pro
 
 int mcg_scene_tune(mcg_scene* scene, const mcg_tune* tune) {
     if (!scene) return mcg::set_error(MCG_EINVAL, "NULL scene");
-    mcg_tune t = tune ? *tune : mcg_tune{0, 0, 0, 0, 0};
+    mcg_tune t = tune ? *tune : mcg_tune{};
     if (t.block != 0 && (t.block < 64 || t.block > 1024 || t.block % 64 != 0))
         return mcg::set_error(MCG_EINVAL, "block must be a multiple of 64 from 64 to 1024");
-    if (t.chunk < 0 || t.blocks_per_cu < 0 || t.per_thread < 0)
+    if (t.chunk < 0 || t.blocks_per_cu < 0 || t.per_thread < 0 || t.compact_every < 0 || t.live < 0)
         return mcg::set_error(MCG_EINVAL, "negative tuning value");
+    if (t.per_thread != 0 && t.per_thread != 1 && t.per_thread != 2 && t.per_thread != 4 && t.per_thread != 8)
+        return mcg::set_error(MCG_EINVAL, "per_thread must be 1, 2, 4 or 8");
+    if (t.engine != MCG_ENGINE_HISTORY && t.engine != MCG_ENGINE_EVENT)
+        return mcg::set_error(MCG_EINVAL, "unknown engine");
     if (t.chunk != 0 && t.chunk < 64) return mcg::set_error(MCG_EINVAL, "chunk must be at least 64");
     scene->tune = t;
     return MCG_OK;
@@ -2619,6 +3175,9 @@ int mcg_solve(mcg_scene* scene, const mcg_problem_spec* prob, double* out, int64
         if (stats) *stats = mcg_stats{};
         return MCG_OK;
     }
+    if (scene->tune.engine == MCG_ENGINE_EVENT)
+        return p.precision == MCG_FP64 ? mcg::run_solve_event<double>(scene, p, out, counts, stats)
+                                       : mcg::run_solve_event<float>(scene, p, out, counts, stats);
     if (p.precision == MCG_FP64) return mcg::run_solve<double>(scene, p, out, counts, stats);
     return mcg::run_solve<float>(scene, p, out, counts, stats);
 }

@@ -40,7 +40,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert engine.lib().mcg_abi_version() == 1
+    """The header's version, and the ctypes mirror of mcg_tune has the
+    header's size (version 2 added the engine fields)."""
+    assert engine.lib().mcg_abi_version() == 2
+    text = open(os.path.join(ROOT, "include", "mcg.h")).read()
+    assert re.search(r"#define MCG_ABI_VERSION 2\b", text)
+    assert C.sizeof(abi.Tune) == 40
 
 
 def test_host_entry_points_without_gpu():
