@@ -53,7 +53,8 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_STEP_FP32 = 96   # 48 B SoA bundle state read + written per scatter-step
 BYTES_PER_STEP_FP64 = 152  # 76 B state in FP64
 NBUNDLES = 10_000_000
-CPU_SAMPLE = 2_000_000
+CPU_REPS = 3
+KERNEL_NAME = "trajectory_kernel<float, true, true, 0>"   # <F, tables in LDS, tally in LDS, walk axes>
 
 
 def workload():
@@ -71,6 +72,29 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+
+
+def cpu_median(solves, reps=CPU_REPS):
+    """The FP64 checker over (material, domain, problem) solves, reps times:
+    the median rate over the reps, each rep's time being the checker's own
+    solve timer (steady_clock around the solve, tables and set-up excluded)
+    summed over the solves; the checker runs OpenMP static over bundles."""
+    from oracle import oracle  # CPU baseline leg only
+    rates, steps, secs = [], 0, []
+    for _ in range(reps):
+        t = 0.0
+        steps = 0
+        for mat, dom, prob in solves:
+            _, cst, _ = oracle.solve(mat, dom, prob, threads=cpu_threads())
+            t += cst["kernel_ms"] * 1e-3
+            steps += cst["steps"]
+        secs.append(t)
+        rates.append(steps / t)
+    return float(np.median(rates)), int(steps), [round(x, 3) for x in secs]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,14 +109,18 @@ def main():
                     default=os.path.join(ROOT, "profiles", "trajectory_kernel_bar1e7_fp32_traffic.json"),
                     help="counter-measured HBM bytes per launch of the kernel (rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE, scaled by the copy-probe calibration)")
-    ap.add_argument("--only", choices=("slab", "porous_film", "octet", "batched", "serving"),
+    ap.add_argument("--quick", action="store_true",
+                    help="with --only engines: one launch shape per engine, one timed solve each")
+    ap.add_argument("--only", choices=("slab", "porous_film", "octet", "batched", "serving", "engines"),
                     help="run one secondary workload alone and print its JSON object")
     args = ap.parse_args()
     if args.only:
         run = {"slab": lambda: bench_slab(cpu=not args.no_cpu),
                "porous_film": lambda: bench_porous(cpu=not args.no_cpu),
                "octet": lambda: bench_octet(cpu=not args.no_cpu),
-               "batched": bench_batched, "serving": bench_serving}[args.only]
+               "batched": lambda: bench_batched(cpu=not args.no_cpu),
+               "serving": lambda: bench_serving(cpu=not args.no_cpu),
+               "engines": lambda: bench_engines(quick=args.quick)}[args.only]
         print(json.dumps({args.only: run()}))
         return
     if args.copy_probe:
@@ -122,17 +150,30 @@ def main():
         steps = 0
         kern_ms = []
         fields = []
+        lost = 0
         t0 = time.perf_counter()
         for k in range(args.steps):
             f, st, _ = sc.solve(scenes.problem(NBUNDLES, seed=seed + k, precision=abi.FP32_FLIGHT))
             steps += st["steps"]
+            lost += st["escapes"]
             kern_ms.append(st["kernel_ms"])
             fields.append(f)
         elapsed = time.perf_counter() - t0
         progress(f"headline: {args.steps} solves in {elapsed:.2f} s")
-        # precision split: the same streams in FP64
-        f64, st64, _ = sc.solve(scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64))
-        progress("FP64 solve done")
+        # the reference's precision: the same K solves in FP64 (the
+        # deterministic mode), timed the same way; the first is the
+        # precision split's comparison (same streams as the first FP32 solve)
+        sc.solve(scenes.problem(NBUNDLES, seed=999, precision=abi.FP64))
+        steps64 = 0
+        f64 = None
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            f, st64, _ = sc.solve(scenes.problem(NBUNDLES, seed=seed + k, precision=abi.FP64))
+            steps64 += st64["steps"]
+            if f64 is None:
+                f64 = f
+        elapsed64 = time.perf_counter() - t0
+        progress(f"FP64: {args.steps} solves in {elapsed64:.2f} s")
 
     value = steps / elapsed
     ms_per_step = elapsed / args.steps * 1e3
@@ -157,25 +198,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle  # CPU baseline leg only
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-        cprob = scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64, count=CPU_SAMPLE)
-        t1 = time.perf_counter()
-        _, cst, _ = oracle.solve(mat, dom, cprob, threads=threads)
-        ct = time.perf_counter() - t1
-        cpu = {"value": cst["steps"] / ct, "unit": "scatter-steps/s", "cores": threads,
-               "kind": "port",
-               "sample": f"first {CPU_SAMPLE} of the {NBUNDLES} bundles of the same bar "
-                         f"workload, FP64, OpenMP static schedule over bundles "
-                         f"({cst['steps']} scatter-steps in {ct:.2f} s)"}
-        progress(f"CPU baseline done in {ct:.1f} s")
+        rate, csteps, secs = cpu_median([(mat, dom, scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64))])
+        cpu = {"value": rate, "unit": "scatter-steps/s", "cores": cpu_threads(), "kind": "port",
+               "sample": f"the whole {NBUNDLES}-bundle bar solve (seed {seed}), FP64, OpenMP static schedule "
+                         f"over bundles, median of {CPU_REPS} (solve times {secs} s, {csteps} scatter-steps each)"}
+        progress(f"CPU baseline done ({secs} s)")
 
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
         with open(args.traffic_json) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    batched = serving = porous = slab = octet_ = None
+    batched = serving = porous = slab = octet_ = engines = None
     if not args.headline_only:
         slab = bench_slab(cpu=not args.no_cpu)
         progress("slab done")
@@ -183,10 +217,12 @@ def main():
         progress("porous film done")
         octet_ = bench_octet(cpu=not args.no_cpu)
         progress("octet done")
-        batched = bench_batched()
+        batched = bench_batched(cpu=not args.no_cpu)
         progress("batched done")
-        serving = bench_serving()
+        serving = bench_serving(cpu=not args.no_cpu)
         progress("serving done")
+        engines = bench_engines()
+        progress("engines done")
 
     line = {
         "metric": METRIC,
@@ -207,8 +243,11 @@ def main():
                         "FP32 flight + FP64 tallies (one step = one full solve)",
             "bundles": NBUNDLES,
             "scatter_steps_per_solve": steps_per_launch,
-            "fp64_scatter_steps_per_s": st64["steps"] / (st64["kernel_ms"] * 1e-3),
+            "escapes": lost,
+            "escape_fraction": lost / (NBUNDLES * args.steps),
         },
+        "fp64": {"scatter_steps_per_s": steps64 / elapsed64, "ms_per_step": elapsed64 / args.steps * 1e3,
+                 "steps": args.steps, "note": "the same K solves (seeds) in FP64, the deterministic mode"},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved / 1e9,
@@ -216,7 +255,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": "trajectory_kernel<float, true, true>",
+            "kernel": KERNEL_NAME,
             "kernel_ms_mean": mean_kern_s * 1e3,
             "bytes_per_step": BYTES_PER_STEP_FP32,
         },
@@ -227,11 +266,12 @@ def main():
         "octet": octet_,
         "batched": batched,
         "serving": serving,
+        "engines": engines,
     }
     print(json.dumps(line))
 
 
-def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
+def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=1_000_000):
     """Periodic porous film (cylindrical pore, 64 x 64 map), 1e8 bundles,
     FP32 flight, maxscat 100: one full solve per timed step."""
     from montecarlocpp_amd import workloads
@@ -259,6 +299,7 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
                        "x (dT 1 K) and y, diffuse top/bottom, Si-like at 300 K, 1e8 bundles, "
                        "64x64 Multi map, FP32 flight, maxscat 100",
            "bundles": nbundles, "scatter_steps_per_solve": steps / len(runs),
+           "escapes": int(sum(r[2]["escapes"] for r in runs)),
            "steps_per_s": steps / wall, "kernel_steps_per_s": steps / kern,
            "ms_per_solve": wall / len(runs) * 1e3,
            "k_eff_W_per_mK": float(np.mean(keff)), "k_eff_seed_spread": float(abs(keff[0] - keff[1])),
@@ -276,15 +317,11 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=100_000):
                "temperature_map_seed1_vs_seed2": rel(f_warm32[0], f_seed2[0])},
            "roofline_frac_96B": steps / kern * BYTES_PER_STEP_FP32 / HBM_PEAK}
     if cpu:
-        from oracle import oracle  # CPU baseline leg only
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
         cp = workloads.porous_film(cpu_sample, precision=abi.FP64)[2]
-        t0 = time.perf_counter()
-        _, cst, _ = oracle.solve(mat, dom, cp, threads=threads)
-        ct = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s",
-                               "cores": threads, "kind": "port",
-                               "sample": f"{cpu_sample} bundles of the same porous film, FP64"}
+        rate, csteps, secs = cpu_median([(mat, dom, cp)])
+        out["cpu_baseline"] = {"value": rate, "unit": "scatter-steps/s", "cores": cpu_threads(), "kind": "port",
+                               "sample": f"a {cpu_sample}-bundle solve of the same porous film, FP64, median of "
+                                         f"{CPU_REPS} (solve times {secs} s, {csteps} scatter-steps each)"}
     return out
 
 
@@ -308,13 +345,11 @@ def bench_slab(cpu=True, reps=20):
            "steps_per_s": st["steps"] / wall}
     if cpu:
         from oracle import oracle  # CPU baseline leg only
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-        t0 = time.perf_counter()
-        fo, cst, _ = oracle.solve(mat, dom, prob, threads=threads)
-        ct = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s", "cores": threads,
-                               "kind": "port", "sample": "the whole 1e5-bundle solve, FP64",
-                               "ms_per_solve": ct * 1e3}
+        rate, csteps, secs = cpu_median([(mat, dom, prob)])
+        fo, _, _ = oracle.solve(mat, dom, prob, threads=cpu_threads())
+        out["cpu_baseline"] = {"value": rate, "unit": "scatter-steps/s", "cores": cpu_threads(), "kind": "port",
+                               "sample": f"the whole 1e5-bundle solve, FP64, median of {CPU_REPS}",
+                               "ms_per_solve": float(np.median(secs)) * 1e3}
         out["gpu_vs_cpu_tally_rel_diff"] = rel(f, fo)
     return out
 
@@ -337,31 +372,37 @@ def bench_octet(cpu=True, nbundles=10_000_000, cpu_sample=200_000):
             out[key] = {"scatter_steps": st["steps"], "kernel_ms": st["kernel_ms"],
                         "steps_per_s": st["steps"] / (st["kernel_ms"] * 1e-3), "escapes": st["escapes"]}
     if cpu:
-        from oracle import oracle  # CPU baseline leg only
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
         prob = scenes.problem(nbundles, maxscat=100, seed=1, count=cpu_sample)
-        t0 = time.perf_counter()
-        _, cst, _ = oracle.solve(mat, dom, prob, threads=threads)
-        ct = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": cst["steps"] / ct, "unit": "scatter-steps/s", "cores": threads,
-                               "kind": "port", "sample": f"first {cpu_sample} of the 1e7 bundles, FP64"}
+        rate, csteps, secs = cpu_median([(mat, dom, prob)])
+        out["cpu_baseline"] = {"value": rate, "unit": "scatter-steps/s", "cores": cpu_threads(), "kind": "port",
+                               "sample": f"first {cpu_sample} of the 1e7 bundles, FP64, median of {CPU_REPS} "
+                                         f"(solve times {secs} s)"}
     return out
 
 
-def bench_batched():
+def bench_batched(cpu=True):
     """256 independent slabs (random thickness, T_ref, lateral specularity,
-    bundle count): one persistent launch vs one mcg_solve per slab."""
+    bundle count): one persistent launch with the autotuned batch shape, its
+    graph replay, and one mcg_solve per slab; beside the whole batch on the
+    FP64 checker (one solve per slab, median of 3)."""
     from montecarlocpp_amd import workloads
     cases = workloads.slab_batch(256, precision=abi.FP32_FLIGHT)
     scs = [engine.Scene(m, d) for m, d, _ in cases]
     probs = [p for _, _, p in cases]
-    engine.batch_solve(scs, probs)
+    tuned, tuned_rate = engine.autotune_batch(scs, probs, sample_fraction=1.0)
+    engine.batch_solve(scs, probs, tuned)
     t0 = time.perf_counter()
-    _, st = engine.batch_solve(scs, probs)
+    _, st = engine.batch_solve(scs, probs, tuned)
     tb = time.perf_counter() - t0
     steps = sum(x["steps"] for x in st)
+    lost = sum(x["escapes"] for x in st)
+    # the default shape, for comparison
+    engine.batch_solve(scs, probs)
+    t0 = time.perf_counter()
+    _, std = engine.batch_solve(scs, probs)
+    td = time.perf_counter() - t0
     # the same batch captured once into a HIP graph and replayed
-    with engine.Batch(scs, probs) as gb:
+    with engine.Batch(scs, probs, tuned) as gb:
         gb.run()
         t0 = time.perf_counter()
         _, gst = gb.run()
@@ -374,24 +415,134 @@ def bench_batched():
     ts = time.perf_counter() - t0
     for sc in scs:
         sc.close()
-    return {"workload": "256 slabs, L log-uniform 10 nm-10 um, T_ref 100-500 K, dT 2 K, "
-                        "lateral specularity U[0,1], 1e4-1e6 bundles each (5.4e7 total), FP32 flight",
-            "bundles": int(sum(p.nemit for p in probs)), "scatter_steps": int(steps),
-            "batched_steps_per_s": steps / tb, "batched_ms": tb * 1e3,
-            "batched_kernel_ms": st[0]["kernel_ms"],
-            "graph_replay_steps_per_s": gsteps / tg, "graph_replay_ms": tg * 1e3,
-            "single_solve_path_steps_per_s": steps / ts, "single_solve_path_ms": ts * 1e3,
-            "single_solve_path_kernel_ms": single_kernel_ms,
-            "speedup_vs_single_solve_path": ts / tb}
+    out = {"workload": "256 slabs, L log-uniform 10 nm-10 um, T_ref 100-500 K, dT 2 K, "
+                       "lateral specularity U[0,1], 1e4-1e6 bundles each (5.4e7 total), FP32 flight",
+           "bundles": int(sum(p.nemit for p in probs)), "scatter_steps": int(steps), "escapes": int(lost),
+           "autotuned_shape": {"block": tuned.block, "chunk": tuned.chunk, "lds_tables": tuned.lds_tables,
+                               "sample_steps_per_s": tuned_rate},
+           "batched_steps_per_s": steps / tb, "batched_ms": tb * 1e3,
+           "batched_kernel_ms": st[0]["kernel_ms"],
+           "default_shape_steps_per_s": sum(x["steps"] for x in std) / td,
+           "graph_replay_steps_per_s": gsteps / tg, "graph_replay_ms": tg * 1e3,
+           "single_solve_path_steps_per_s": steps / ts, "single_solve_path_ms": ts * 1e3,
+           "single_solve_path_kernel_ms": single_kernel_ms,
+           "speedup_vs_single_solve_path": ts / tb}
+    if cpu:
+        c64 = workloads.slab_batch(256, precision=abi.FP64)
+        rate, csteps, secs = cpu_median(c64)
+        out["cpu_baseline"] = {"value": rate, "unit": "scatter-steps/s", "cores": cpu_threads(), "kind": "port",
+                               "sample": f"all 256 slab solves in FP64, one checker solve each, median of "
+                                         f"{CPU_REPS} (batch times {secs} s, {csteps} scatter-steps)"}
+    return out
 
 
-def bench_serving(lam=10.0, nreq=150):
-    """Open-loop Poisson stream of slab / bar / random-slab / porous-film requests
-    (1e4-1e8 bundles, T_ref 100-500 K) through the continuous-batching server."""
+def bench_engines(reps=2, quick=False):
+    """History kernel against the event engine (structure-of-arrays state in
+    HBM, bundles regrouped by next event and cell kind each round) on the bar,
+    the porous film and the octet truss, FP32 flight: the best of `reps`
+    kernel times per launch shape, over a sweep of the event engine's knobs
+    (bundles in flight, bundles per thread, rounds between regroupings)."""
+    from montecarlocpp_amd import octet, workloads
+    cases = {
+        "bar_1e7": (materials.si_like(300.0), scenes.bar(1e-6, 1e-7, 1e-7, 100, t_hot=10.0, t_cold=-10.0),
+                    scenes.problem(10_000_000, seed=1, precision=abi.FP32_FLIGHT)),
+        "porous_2e7": workloads.porous_film(20_000_000),
+        "octet_1e7": (materials.si_like(300.0), octet.octet((6e-7, 1e-7, 1e-7, 1e-8), (4, 2, 2, 1), 1.0),
+                      scenes.problem(10_000_000, maxscat=100, seed=1, precision=abi.FP32_FLIGHT)),
+    }
+    shapes = [(1 << 20, 1, 1), (1 << 21, 1, 1), (1 << 22, 1, 1), (1 << 21, 4, 1), (1 << 21, 1, 4), (1 << 21, 4, 4)]
+    if quick:   # one shape per engine, one timed solve each (for profiling)
+        shapes, reps = [(1 << 22, 1, 1)], 1
+    out = {}
+    for name, (mat, dom, prob) in cases.items():
+        row = {}
+        with engine.Scene(mat, dom) as sc:
+            sc.solve(prob)
+            best = None
+            for _ in range(reps):
+                _, st, _ = sc.solve(prob)
+                best = st if best is None or st["kernel_ms"] < best["kernel_ms"] else best
+            row["history"] = {"kernel_ms": best["kernel_ms"], "steps_per_s": best["steps"] / (best["kernel_ms"] * 1e-3)}
+            ev = []
+            for live, pt, ce in shapes:
+                t = abi.Tune()
+                t.engine, t.live, t.per_thread, t.compact_every = abi.ENGINE_EVENT, live, pt, ce
+                sc.set_tune(t)
+                sc.solve(prob)
+                best = None
+                for _ in range(reps):
+                    _, st, _ = sc.solve(prob)
+                    best = st if best is None or st["kernel_ms"] < best["kernel_ms"] else best
+                ev.append({"live": live, "per_thread": pt, "compact_every": ce, "kernel_ms": best["kernel_ms"],
+                           "steps_per_s": best["steps"] / (best["kernel_ms"] * 1e-3)})
+            sc.set_tune(None)
+        row["event"] = ev
+        row["event_best_vs_history"] = max(e["steps_per_s"] for e in ev) / row["history"]["steps_per_s"]
+        out[name] = row
+        progress(f"engines {name}: history {row['history']['steps_per_s']:.3g}, "
+                 f"event best {max(e['steps_per_s'] for e in ev):.3g} steps/s")
+    return out
+
+
+def bench_serving(lams=(10, 25, 50, 100, 200, 500), nreq=100, cpu=True, cpu_cap=200_000):
+    """Open-loop Poisson streams of slab / bar / random-slab / porous-film
+    requests (1e4-1e8 bundles, T_ref 100-500 K) through the
+    continuous-batching server, one stream of nreq requests per offered load
+    lambda.  1 % of each stream (at least one request) is checked: its served
+    FP32-flight result against the GPU's FP64 rerun of the whole request (same
+    streams: the precision error), and its first cpu_cap bundles solved in
+    FP64 by the CPU checker and by the GPU (the same bundles: agreement of the
+    GPU's FP64 path with the checker); the error against a full FP64 CPU
+    rerun is the first plus the second."""
     from montecarlocpp_amd import serve
-    out = serve.run(lam, nreq, factory=serve.RequestFactory(), rerun_fraction=0.01)
-    out["requests_mix"] = ("slab / bar / random slab / porous film, bundles log-uniform 1e4-1e8, "
-                           "FP32 flight")
+    fac = serve.RequestFactory()
+    sweep = []
+    checks = []
+    with engine.Server() as sv:
+        for lam in lams:
+            r = serve.run(lam, nreq, server=sv, factory=fac, seed=int(lam), rerun_fraction=0.01,
+                          keep_sampled=True)
+            sampled = r.pop("_sampled", [])
+            sweep.append(r)
+            progress(f"serving lambda {lam}: {r['solves_per_s']:.1f} solves/s, "
+                     f"p50 {r['latency_p50_s'] * 1e3:.1f} ms, p99 {r['latency_p99_s']:.2f} s")
+            for (fam, mat, dom, prob, served), err in zip(sampled, r["fp64_rerun_errors"]):
+                checks.append((lam, fam, mat, dom, prob, err))
+    # the batch kernel on the same mix: the requests of one stream (the ids of
+    # every stream are 1..nreq) as one mcg_batch_solve
+    reqs = [fac.make(r + 1) for r in range(nreq)]
+    scs = [engine.Scene(m, d) for _, m, d, _ in reqs]
+    probs = [p for _, _, _, p in reqs]
+    t0 = time.perf_counter()
+    _, bst = engine.batch_solve(scs, probs)
+    tb = time.perf_counter() - t0
+    for sc in scs:
+        sc.close()
+    mix_rate = sum(x["steps"] for x in bst) / tb
+    saturated = max(r["scatter_steps_per_s"] for r in sweep)
+    out = {"requests_mix": "slab / bar / random slab / porous film, bundles log-uniform 1e4-1e8, "
+                           "T_ref 100-500 K, FP32 flight; one server, streams run back to back",
+           "requests_per_lambda": nreq, "sweep": sweep,
+           "same_requests_one_batch_steps_per_s": mix_rate,
+           "saturated_served_vs_batch": saturated / mix_rate}
+    if cpu:
+        from oracle import oracle  # checker leg, after the timed streams
+        rows = []
+        for lam, fam, mat, dom, prob, err in checks:
+            n = min(int(prob.nemit), cpu_cap)
+            p64 = scenes.problem(prob.nemit, tally=prob.tally, maxscat=prob.maxscat, maxloop=prob.maxloop,
+                                 seed=prob.seed, precision=abi.FP64, size=prob.size, count=n)
+            fc, _, _ = oracle.solve(mat, dom, p64)
+            with engine.Scene(mat, dom) as sc:
+                fg, _, _ = sc.solve(p64)
+            gc = max(float(np.linalg.norm(fg[k] - fc[k]) / np.linalg.norm(fc[k])) for k in range(fc.shape[0])
+                     if np.linalg.norm(fc[k]) > 0)
+            rows.append({"lambda_per_s": lam, "family": fam, "bundles": int(prob.nemit), "cpu_bundles": n,
+                         "served_fp32_vs_gpu_fp64": {k: err[k] for k in ("temperature", "heat_flux",
+                                                                         "temperature_in_se", "heat_flux_in_se")},
+                         "gpu_fp64_vs_cpu_fp64_same_bundles": gc,
+                         "vs_cpu_fp64_bound": max(err["temperature"], err["heat_flux"]) + gc})
+        out["cpu_fp64_checks"] = rows
     return out
 
 

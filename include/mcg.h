@@ -385,12 +385,15 @@ int mcg_server_info(const mcg_server* server, mcg_server_stats* out);
 int mcg_server_destroy(mcg_server* server);
 
 /* ------------------------------------------------------------- autotuner
- * Times candidate launch shapes (block 64-1024, sampling tables in LDS or
- * not, bundle ids per wavefront grab) on a bounded sample of
- * the problem's bundles and returns the fastest; mcg_autotune also installs
- * it on the scene (mcg_scene_tune).  Only scheduling changes, never results.
- * sample_bundles <= 0 means min(nemit, 2^20); sample_fraction <= 0 means 0.1.
- * Phonons carried per thread is fixed at 1 in this build. */
+ * Times candidate shapes on a bounded sample of the problem's bundles and
+ * returns the fastest; mcg_autotune also installs it on the scene
+ * (mcg_scene_tune).  mcg_autotune's candidates are both engines: the history
+ * kernel's block (64-1024), table placement (LDS or not) and bundle ids per
+ * wavefront grab, and the event engine's bundles in flight, bundles per
+ * thread (1, 4) and rounds between regroupings (1, 4).  mcg_autotune_batch
+ * tunes the batch kernel's block, table placement and grab.  Only
+ * scheduling changes, never results.  sample_bundles <= 0 means
+ * min(nemit, 2^20); sample_fraction <= 0 means 0.1. */
 int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_bundles,
                  mcg_tune* best, double* best_steps_per_s);
 int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_spec* probs,

@@ -1432,6 +1432,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(const KParams<F>* __restric
 // to FP64 summation order).
 constexpr int kEvGroups = 6;
 constexpr int32_t kEvHole = -1000;   // pending-event mark of a slot left empty in place
+constexpr int kEvPad = 32;           // counters 256 B apart: one cache line each
 
 template <typename F>
 struct EvState {
@@ -1447,8 +1448,8 @@ template <typename F>
 struct EvParams {
     KParams<F> K;
     EvState<F> in, out;                 // kEvGroups regions of `cap` slots each
-    unsigned long long* in_count;       // entries per group of the input list
-    unsigned long long* out_count;      // entries per group of the output list
+    unsigned long long* in_count;       // entries per group of the input list (kEvPad apart)
+    unsigned long long* out_count;      // entries per group of the output list (kEvPad apart)
     unsigned long long* emit;           // [0] first id offset, [1] bundles to emit this round
     int64_t cap;
     int64_t live_target;
@@ -1502,22 +1503,36 @@ __device__ __forceinline__ int64_t ev_slot(const unsigned long long (&cnt)[kEvGr
     return -1;
 }
 
-// Append the lanes with `want` to group `grp` of a list: one atomic per group
-// and wavefront; returns the lane's slot (or -1).  All lanes call.
+// Append the lanes with `want` to group `grp` of a list; returns the lane's
+// slot (or -1).  The wavefronts of a workgroup reserve their places in LDS
+// and one thread per group reserves the workgroup's in the list, so a
+// workgroup costs one global atomic per group.  Every thread of the
+// workgroup calls (three barriers).
+struct EvAppendLds {
+    unsigned int cnt[kEvGroups];
+    unsigned long long base[kEvGroups];
+};
+
 __device__ __forceinline__ int64_t ev_append(unsigned long long* count, int64_t cap, int grp, bool want, int lane,
-                                             uint64_t lt_mask) {
-    int64_t slot = -1;
+                                             uint64_t lt_mask, EvAppendLds& A) {
+    const int tid = threadIdx.x;
+    if (tid < kEvGroups) A.cnt[tid] = 0u;
+    __syncthreads();
+    unsigned int off = 0;
 #pragma unroll
     for (int g = 0; g < kEvGroups; ++g) {
         const uint64_t m = __ballot(want && grp == g);
         if (m == 0ull) continue;
         const int leader = __ffsll((long long)m) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(count + g, (unsigned long long)popc(m));
-        base = __shfl(base, leader);
-        if (want && grp == g) slot = int64_t(g) * cap + int64_t(base) + popc(m & lt_mask);
+        unsigned int w = 0;
+        if (lane == leader) w = atomicAdd(&A.cnt[g], unsigned(popc(m)));
+        w = __shfl(w, leader);
+        if (want && grp == g) off = w + unsigned(popc(m & lt_mask));
     }
-    return slot;
+    __syncthreads();
+    if (tid < kEvGroups) A.base[tid] = A.cnt[tid] ? atomicAdd(count + tid * kEvPad, (unsigned long long)A.cnt[tid]) : 0ull;
+    __syncthreads();
+    return want ? int64_t(grp) * cap + int64_t(A.base[grp]) + int64_t(off) : -1;
 }
 
 // The next flight of a bundle whose collision left it alive: advect, kill
@@ -1556,6 +1571,7 @@ __device__ __forceinline__ void ev_finish(const KParams<F>& K, const Lane<F>& L,
 template <typename F, bool kLdsTally>
 __global__ void __launch_bounds__(1024) ev_round_kernel(EvParams<F> E) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ EvAppendLds app;
     const KParams<F>& K = E.K;
     const int tid = threadIdx.x;
     const View<F> V = stage<F, false, kLdsTally>(K, lds, tid, blockDim.x);
@@ -1567,7 +1583,7 @@ __global__ void __launch_bounds__(1024) ev_round_kernel(EvParams<F> E) {
     int64_t total = 0;
 #pragma unroll
     for (int g = 0; g < kEvGroups; ++g) {
-        cnt[g] = E.in_count[g];
+        cnt[g] = E.in_count[g * kEvPad];
         total += int64_t(cnt[g]);
     }
     LaneStats st;
@@ -1592,7 +1608,7 @@ __global__ void __launch_bounds__(1024) ev_round_kernel(EvParams<F> E) {
             }
         }
         if (E.regroup) {
-            const int64_t o = ev_append(E.out_count, E.cap, grp, live, lane, lt_mask);
+            const int64_t o = ev_append(E.out_count, E.cap, grp, live, lane, lt_mask, app);
             if (live) ev_store(E.out, o, L, hit);
         } else if (s >= 0) {
             // kept in place: a bundle that ended leaves a hole
@@ -1614,14 +1630,14 @@ __global__ void ev_prep_kernel(unsigned long long* out_count, unsigned long long
                                int64_t live_target) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int64_t live = 0;
-    for (int g = 0; g < kEvGroups; ++g) live += int64_t(out_count[g]);
+    for (int g = 0; g < kEvGroups; ++g) live += int64_t(out_count[g * kEvPad]);
     const int64_t left = count - int64_t(*next_id);
     const int64_t want = live_target > live ? live_target - live : 0;
     const int64_t n = want < left ? want : left;
     emit[0] = *next_id;
     emit[1] = (unsigned long long)(n > 0 ? n : 0);
     *next_id += emit[1];
-    for (int g = 0; g < kEvGroups; ++g) next_count[g] = 0ull;
+    for (int g = 0; g < kEvGroups; ++g) next_count[g * kEvPad] = 0ull;
 }
 
 // Emission of this round's new bundles and their first flight; appended to
@@ -1629,6 +1645,7 @@ __global__ void ev_prep_kernel(unsigned long long* out_count, unsigned long long
 template <typename F, bool kLdsTally>
 __global__ void __launch_bounds__(1024) ev_emit_kernel(EvParams<F> E) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ EvAppendLds app;
     const KParams<F>& K = E.K;
     const int tid = threadIdx.x;
     const View<F> V = stage<F, false, kLdsTally>(K, lds, tid, blockDim.x);
@@ -1655,7 +1672,7 @@ __global__ void __launch_bounds__(1024) ev_emit_kernel(EvParams<F> E) {
                 grp = ev_group(V, L, hit);
             }
         }
-        const int64_t o = ev_append(E.out_count, E.cap, grp, live, lane, lt_mask);
+        const int64_t o = ev_append(E.out_count, E.cap, grp, live, lane, lt_mask, app);
         if (live) ev_store(E.out, o, L, hit);
     }
     st.publish(K.counters, lane);
@@ -1679,7 +1696,7 @@ __global__ void __launch_bounds__(1024) ev_drain_kernel(EvParams<F> E) {
     int64_t total = 0;
 #pragma unroll
     for (int g = 0; g < kEvGroups; ++g) {
-        cnt[g] = E.in_count[g];
+        cnt[g] = E.in_count[g * kEvPad];
         total += int64_t(cnt[g]);
     }
     LaneStats st;
@@ -2287,11 +2304,13 @@ int run_solve_event(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts
     const int64_t slots = int64_t(kEvGroups) * cap;
     const size_t list_bytes = ev_list_bytes<F>(slots);
     if ((rc = sc->ev_state.reserve(2 * list_bytes))) return rc;
-    // counters: list A (kEvGroups), list B (kEvGroups), emit (2), next id (1)
-    if ((rc = sc->ev_ctl.reserve(2 * kEvGroups + 3))) return rc;
+    // counters, each on its own cache line: list A (kEvGroups), list B
+    // (kEvGroups), then emit (2) and the next id (1) together
+    const size_t nctl = size_t(2 * kEvGroups + 1) * kEvPad;
+    if ((rc = sc->ev_ctl.reserve(nctl))) return rc;
     unsigned long long* const cA = sc->ev_ctl.p;
-    unsigned long long* const cB = cA + kEvGroups;
-    unsigned long long* const emitw = cB + kEvGroups;
+    unsigned long long* const cB = cA + kEvGroups * kEvPad;
+    unsigned long long* const emitw = cB + kEvGroups * kEvPad;
     unsigned long long* const next_id = emitw + 2;
     DevBuf<uint32_t> cnt;
     const bool want_counts = counts && counts->nscat && counts->nbdry && counts->nstep && counts->fate;
@@ -2302,8 +2321,8 @@ int run_solve_event(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts
     unsigned long long* const ctr = reinterpret_cast<unsigned long long*>(direct + nd + nf);
     hipLaunchKernelGGL(zero_kernel, dim3(int(std::min<size_t>(1024, (nd + nf + 255) / 256 + 1))), dim3(256), 0, 0,
                        direct, nd + nf, static_cast<double*>(nullptr), size_t(0), ctr, size_t(C_COUNT));
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, 0, static_cast<double*>(nullptr), size_t(0),
-                       static_cast<double*>(nullptr), size_t(0), sc->ev_ctl.p, size_t(2 * kEvGroups + 3));
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, 0, static_cast<double*>(nullptr), size_t(0),
+                       static_cast<double*>(nullptr), size_t(0), sc->ev_ctl.p, nctl);
     HIP_TRY(hipGetLastError());
 
     EvParams<F> E{};
@@ -2325,11 +2344,14 @@ int run_solve_event(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts
     // LDS: geometry, and the tally when it fits (the tables stay in global
     // memory: every launch stages what it uses)
     const size_t geo = lds_bytes<F>(h, rows, false, false);
-    const LdsPlan lp = plan_lds(geo, 0, lds_bytes<F>(h, rows, false, true) - geo, h.has_grid, false, sc->lds_limit);
+    // (256 B of the LDS are the appends' static workspace)
+    const LdsPlan lp = plan_lds(geo, 0, lds_bytes<F>(h, rows, false, true) - geo, h.has_grid, false,
+                                sc->lds_limit - 256);
     if (lp.bytes > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     E.K.lds_tally = lp.tally ? 1 : 0;
-    const int block = sc->tune.block > 0 ? sc->tune.block : 256;
     int per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lp.bytes, 1)));
+    // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
+    const int block = sc->tune.block > 0 ? sc->tune.block : (per_cu == 1 ? 1024 : 256);
     per_cu = std::min(per_cu, std::max(1, 2048 / block));
     if (sc->tune.blocks_per_cu > 0) per_cu = std::min(per_cu, int(sc->tune.blocks_per_cu));
     const int64_t max_grid = int64_t(sc->num_cu) * per_cu;
@@ -2386,12 +2408,12 @@ int run_solve_event(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts
             if ((rc = launch(1, R, grid_for(live)))) return rc;
             a_current = !a_current;
         }
-        unsigned long long hc[2 * kEvGroups + 3];
-        HIP_TRY(hipMemcpy(hc, sc->ev_ctl.p, sizeof(hc), hipMemcpyDeviceToHost));
-        const unsigned long long* cur = a_current ? hc : hc + kEvGroups;
+        std::vector<unsigned long long> hc(nctl);
+        HIP_TRY(hipMemcpy(hc.data(), sc->ev_ctl.p, nctl * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        const unsigned long long* cur = hc.data() + (a_current ? 0 : kEvGroups * kEvPad);
         int64_t in_list = 0;
-        for (int g = 0; g < kEvGroups; ++g) in_list += int64_t(cur[g]);
-        const bool emitted_all = int64_t(hc[2 * kEvGroups + 2]) >= p.count;
+        for (int g = 0; g < kEvGroups; ++g) in_list += int64_t(cur[g * kEvPad]);
+        const bool emitted_all = int64_t(hc[size_t(2 * kEvGroups) * kEvPad + 2]) >= p.count;
         if (emitted_all && in_list <= drain_at) {
             if (in_list > 0) {
                 EvParams<F> R = E;
@@ -2809,9 +2831,14 @@ struct GraphBatch : GraphBatchBase {
 // Steps are pipelined: a call plans and enqueues step k+1 (bundle cursors
 // advance when work is handed out), then waits for step k and retires the
 // requests it finished, so the GPU runs step k+1 while the host copies
-// results, admits arrivals and plans.  The server's stream does not
-// synchronise with the null stream; device memory of retired requests is
-// freed only when the server is idle, because freeing can stall the device.
+// results, admits arrivals and plans.  Consecutive steps go to two streams
+// of the default priority, so step k+1's launch starts on the CUs that step
+// k's last, long-lived bundles leave idle instead of waiting for the whole
+// of step k.  Both launches add into the tallies of a request they share
+// (FP64 atomics); a request retires only once every step holding its
+// bundles has finished.  The streams do not synchronise with the null
+// stream; device memory of retired requests is freed only when the server
+// is idle, because freeing can stall the device.
 namespace mcg {
 
 struct Request {
@@ -2835,12 +2862,20 @@ struct StepSlot {
     DevBuf<KParams<float>> kp32;
     DevBuf<KParams<double>> kp64;
     DevBuf<unsigned long long> nx32, nx64;   // per-request bundle claim counters of the step
+    hipStream_t stream = nullptr;            // this slot's stream (default priority)
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t ez = nullptr;                 // after the step's clearing of newly admitted requests
+    bool ez_set = false;
     bool live = false;
     std::vector<std::pair<Request*, int64_t>> handed;
     ~StepSlot() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (ez) (void)hipEventDestroy(ez);
     }
 };
 
@@ -2856,17 +2891,11 @@ struct mcg_server {
     std::map<int64_t, std::unique_ptr<mcg::Request>> all;
     std::vector<std::unique_ptr<mcg::Request>> graveyard;   // retired, device memory not yet freed
     std::vector<int64_t> completed;                          // ids retired by the last step call
-    hipStream_t stream = nullptr;
+    std::vector<mcg::Request*> fresh;                        // admitted, tallies not yet cleared
     mcg::StepSlot slot[2];
     int cur = 0;                                             // slot of the step in flight
     double kernel_ms_total = 0.0;
     int64_t steps_total = 0;
-    ~mcg_server() {
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
 };
 
 namespace mcg {
@@ -2881,9 +2910,8 @@ int admit(mcg_server* sv, Request* r) {
     if ((rc = r->diff.alloc(nf))) return rc;
     if ((rc = r->ctr.alloc(C_COUNT))) return rc;
     if ((rc = r->cdf.upload(r->P.emit_cdf.data(), r->P.emit_cdf.size()))) return rc;
-    HIP_TRY(hipMemsetAsync(r->direct.p, 0, nd * sizeof(double), sv->stream));
-    HIP_TRY(hipMemsetAsync(r->diff.p, 0, nf * sizeof(double), sv->stream));
-    HIP_TRY(hipMemsetAsync(r->ctr.p, 0, C_COUNT * sizeof(unsigned long long), sv->stream));
+    // cleared on the stream of the first step that uses the request
+    sv->fresh.push_back(r);
     r->state = 1;
     sv->used += r->bytes;
     sv->active.push_back(r);
@@ -2962,12 +2990,13 @@ int enqueue_group(mcg_server* sv, StepSlot& slot, const std::vector<Request*>& r
     if (kpbuf.n < size_t(n) && (rc = kpbuf.alloc(size_t(n) * 2))) return rc;
     // synchronous copy: kpbuf of this slot is not in use on the device
     HIP_TRY(hipMemcpy(kpbuf.p, K.data(), K.size() * sizeof(KParams<F>), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemsetAsync(nxbuf.p, 0, size_t(n) * sizeof(unsigned long long), sv->stream));
-    return launch_batch<F>(B, kpbuf.p, n, sv->stream);
+    HIP_TRY(hipMemsetAsync(nxbuf.p, 0, size_t(n) * sizeof(unsigned long long), slot.stream));
+    return launch_batch<F>(B, kpbuf.p, n, slot.stream);
 }
 
-// Hand out fair shares of the step budget and enqueue the step in `slot`.
-int enqueue_step(mcg_server* sv, StepSlot& slot) {
+// Hand out fair shares of the step budget and enqueue the step in `slot`;
+// `other` is the slot whose step may still run.
+int enqueue_step(mcg_server* sv, StepSlot& slot, StepSlot& other) {
     slot.handed.clear();
     slot.live = false;
     std::vector<Request*> g32, g64;
@@ -2975,21 +3004,38 @@ int enqueue_step(mcg_server* sv, StepSlot& slot) {
     int with_work = 0;
     for (Request* r : sv->active)
         if (r->cursor < r->P.count) ++with_work;
-    if (!with_work) return MCG_OK;
-    const int64_t share = std::max<int64_t>(1 << 16, sv->step_bundles / with_work);
-    for (Request* r : sv->active) {
-        const int64_t take = std::min(share, r->P.count - r->cursor);
-        if (take <= 0) continue;
-        (r->P.precision == MCG_FP64 ? g64 : g32).push_back(r);
-        (r->P.precision == MCG_FP64 ? t64 : t32).push_back(take);
+    if (!with_work && sv->fresh.empty()) return MCG_OK;
+    if (with_work) {
+        const int64_t share = std::max<int64_t>(1 << 16, sv->step_bundles / with_work);
+        for (Request* r : sv->active) {
+            const int64_t take = std::min(share, r->P.count - r->cursor);
+            if (take <= 0) continue;
+            (r->P.precision == MCG_FP64 ? g64 : g32).push_back(r);
+            (r->P.precision == MCG_FP64 ? t64 : t32).push_back(take);
+        }
     }
     int rc;
+    if (!slot.stream) HIP_TRY(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking));
     if (!slot.e0) HIP_TRY(hipEventCreate(&slot.e0));
     if (!slot.e1) HIP_TRY(hipEventCreate(&slot.e1));
-    HIP_TRY(hipEventRecord(slot.e0, sv->stream));
+    if (!slot.ez) HIP_TRY(hipEventCreate(&slot.ez));
+    // the requests the other step cleared are cleared before this step adds
+    // to them (the wait covers the clearing, not the other step's kernel)
+    if (other.ez_set) HIP_TRY(hipStreamWaitEvent(slot.stream, other.ez, 0));
+    HIP_TRY(hipEventRecord(slot.e0, slot.stream));
+    // requests admitted since the last step start from cleared tallies (no
+    // earlier step touches them); the step's end event covers the clearing
+    slot.ez_set = !sv->fresh.empty();
+    for (Request* r : sv->fresh) {
+        HIP_TRY(hipMemsetAsync(r->direct.p, 0, r->direct.n * sizeof(double), slot.stream));
+        HIP_TRY(hipMemsetAsync(r->diff.p, 0, r->diff.n * sizeof(double), slot.stream));
+        HIP_TRY(hipMemsetAsync(r->ctr.p, 0, C_COUNT * sizeof(unsigned long long), slot.stream));
+    }
+    sv->fresh.clear();
+    if (slot.ez_set) HIP_TRY(hipEventRecord(slot.ez, slot.stream));
     if (!g32.empty() && (rc = enqueue_group<float>(sv, slot, g32, t32, slot.kp32, slot.nx32))) return rc;
     if (!g64.empty() && (rc = enqueue_group<double>(sv, slot, g64, t64, slot.kp64, slot.nx64))) return rc;
-    HIP_TRY(hipEventRecord(slot.e1, sv->stream));
+    HIP_TRY(hipEventRecord(slot.e1, slot.stream));
     slot.live = true;
     return MCG_OK;
 }
@@ -3381,30 +3427,59 @@ int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_
     const mcg_tune saved = scene->tune;
     mcg_tune top = saved;
     double top_rate = -1.0;
+    // candidates: the history kernel's block, table placement and grab size;
+    // the event engine's bundles in flight, bundles per thread and rounds
+    // between regroupings
+    std::vector<mcg_tune> cand;
     static const int blocks[] = {64, 128, 256, 512, 1024};
     static const int tables[] = {1, -1};
     static const int chunks[] = {64, 256, 1024};
     for (int b : blocks)
         for (int t : tables)
             for (int c : chunks) {
-                scene->tune = mcg_tune{b, 1, c, 0, t};
-                mcg_stats st{};
-                double ms = 1e300;
-                for (int rep = 0; rep < 2; ++rep) {   // first run warms up the shape
-                    rc = p.precision == MCG_FP64 ? mcg::run_solve<double>(scene, p, out.data(), nullptr, &st)
-                                                 : mcg::run_solve<float>(scene, p, out.data(), nullptr, &st);
-                    if (rc) {
-                        scene->tune = saved;
-                        return rc;
-                    }
-                    ms = std::min(ms, st.kernel_ms);
-                }
-                const double rate = double(st.steps) / (ms * 1e-3);
-                if (rate > top_rate) {
-                    top_rate = rate;
-                    top = scene->tune;
-                }
+                mcg_tune x{};
+                x.block = b;
+                x.chunk = c;
+                x.lds_tables = t;
+                x.engine = MCG_ENGINE_HISTORY;
+                cand.push_back(x);
             }
+    static const int64_t lives[] = {int64_t(1) << 18, int64_t(1) << 20};
+    static const int per_thread[] = {1, 4};
+    static const int compact[] = {1, 4};
+    for (int64_t lv : lives)
+        for (int pt : per_thread)
+            for (int ce : compact) {
+                mcg_tune x{};
+                x.engine = MCG_ENGINE_EVENT;
+                x.live = std::min<int64_t>(lv, n);
+                x.per_thread = pt;
+                x.compact_every = ce;
+                cand.push_back(x);
+            }
+    for (const mcg_tune& x : cand) {
+        scene->tune = x;
+        mcg_stats st{};
+        double ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {   // first run warms up the shape
+            if (x.engine == MCG_ENGINE_EVENT)
+                rc = p.precision == MCG_FP64 ? mcg::run_solve_event<double>(scene, p, out.data(), nullptr, &st)
+                                             : mcg::run_solve_event<float>(scene, p, out.data(), nullptr, &st);
+            else
+                rc = p.precision == MCG_FP64 ? mcg::run_solve<double>(scene, p, out.data(), nullptr, &st)
+                                             : mcg::run_solve<float>(scene, p, out.data(), nullptr, &st);
+            if (rc) {
+                scene->tune = saved;
+                return rc;
+            }
+            ms = std::min(ms, st.kernel_ms);
+        }
+        const double rate = double(st.steps) / (ms * 1e-3);
+        if (rate > top_rate) {
+            top_rate = rate;
+            top = x;
+        }
+    }
     scene->tune = top;
     if (best) *best = top;
     if (best_steps_per_s) *best_steps_per_s = top_rate;
@@ -3431,7 +3506,7 @@ int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_sp
         optr[i] = outs[i].data();
     }
     std::vector<mcg_stats> st(n);
-    mcg_tune top{0, 1, 0, 0, 0};
+    mcg_tune top{};
     double top_rate = -1.0;
     static const int blocks[] = {128, 256, 512, 1024};
     static const int tables[] = {1, -1};
@@ -3439,7 +3514,10 @@ int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_sp
     for (int b : blocks)
         for (int t : tables)
             for (int c : chunks) {
-                const mcg_tune cand{b, 1, c, 0, t};
+                mcg_tune cand{};
+                cand.block = b;
+                cand.chunk = c;
+                cand.lds_tables = t;
                 double ms = 1e300;
                 int64_t steps = 0;
                 for (int rep = 0; rep < 2; ++rep) {
@@ -3475,7 +3553,7 @@ int mcg_server_create(uint64_t hbm_budget_bytes, int64_t step_bundles, const mcg
     sv->budget = hbm_budget_bytes > 0 ? size_t(hbm_budget_bytes) : size_t(200) << 30;
     sv->step_bundles = step_bundles > 0 ? step_bundles : (int64_t(1) << 24);
     if (tune) sv->tune = *tune;
-    HIP_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+    for (mcg::StepSlot& sl : sv->slot) HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
     *out = sv.release();
     return MCG_OK;
 }
@@ -3515,7 +3593,7 @@ int mcg_server_step(mcg_server* sv, int32_t* active, int32_t* queued, int32_t* c
     }
     // enqueue the next step behind the one in flight, then finish that one
     const int nxt = 1 - sv->cur;
-    if ((rc = mcg::enqueue_step(sv, sv->slot[nxt]))) return rc;
+    if ((rc = mcg::enqueue_step(sv, sv->slot[nxt], sv->slot[sv->cur]))) return rc;
     if ((rc = mcg::finish_step(sv, sv->slot[sv->cur]))) return rc;
     sv->cur = nxt;
     if (!sv->slot[sv->cur].live) {
@@ -3569,6 +3647,7 @@ int mcg_server_cancel(mcg_server* sv, int64_t id) {
         if (rc) return rc;
         if (r->state == 1) {
             sv->active.erase(std::remove(sv->active.begin(), sv->active.end(), r), sv->active.end());
+            sv->fresh.erase(std::remove(sv->fresh.begin(), sv->fresh.end(), r), sv->fresh.end());
             sv->used -= std::min(sv->used, r->bytes);
         }
     }
@@ -3590,7 +3669,8 @@ int mcg_server_info(const mcg_server* sv, mcg_server_stats* out) {
 
 int mcg_server_destroy(mcg_server* sv) {
     if (!sv) return MCG_OK;
-    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+    for (mcg::StepSlot& sl : sv->slot)
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     mcg::bury(sv);
     delete sv;
     return MCG_OK;

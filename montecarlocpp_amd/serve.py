@@ -3,9 +3,11 @@
 Requests arrive as a Poisson stream of rate lam (solves/s).  Request id r
 draws, from the Philox block (counter (r, 0, 0, 0), key (0x5EED0004, 0)) as
 uniforms u0..u3: its family (u0), its bundle count N = 10^(log10 nmin + (log10 nmax -
-log10 nmin) u1) and its T_ref = 100 + 400 u2 K; its seed is r.  The
-families are the slab, the bar, a random slab of the batched workload and the
-periodic porous film.  Requests are
+log10 nmin) u1), its T_ref = 100 + 400 u2 K and, for a random slab, its
+thickness 10^(-8 + 3 u3) m; the random slab's lateral specularity is the
+first word of the block (r, 1, 0, 0).  Its seed is r.  The families are the
+slab, the bar, a random slab of the batched workload and the periodic porous
+film.  Requests are
 submitted to an mcg_server as their arrival time passes and the server is
 stepped continuously; a request's latency runs from its scheduled arrival to
 the step that finished it.
@@ -53,11 +55,10 @@ class RequestFactory:
         elif fam == "bar":
             dom = scenes.bar(1e-6, 1e-7, 1e-7, 100, 10.0, -10.0)
         else:
-            prm = workloads.slab_batch_params(1)[0]
             L = 10.0 ** (-8.0 + 3.0 * u[3])
             bins = int(round(20 + 180 * math.log10(L / 1e-8) / 3))
-            dom = scenes.slab(L, bins, side=abi.PARTIAL, specularity=u[3])
-            del prm
+            spec = engine.philox([rid, 1, 0, 0], list(REQUEST_KEY))[0] / 2.0 ** 32
+            dom = scenes.slab(L, bins, side=abi.PARTIAL, specularity=spec)
         if fam == "porous_film":
             dom = scenes.porous_film(workloads.PORE_DIM, workloads.PORE_RADIUS, (64, 64, 0))
             prob = scenes.problem(n, seed=rid, precision=self.precision,
@@ -75,28 +76,60 @@ def _rownorm_rel(a, b):
     return out
 
 
+RERUN_PARTS = 8
+
+
 def rerun_errors(reqs, results, ids):
     """Tally error of served requests against FP64 reruns of the same
-    problems (same seed and streams) through mcg_solve: row-norm relative
-    error of the temperature row and of the flux rows of the Multi tally."""
+    problems (same seed and streams) through mcg_solve.  The rerun is solved
+    as RERUN_PARTS bundle sub-ranges, whose sum is the FP64 solve and whose
+    spread gives its standard error per cell.  Reported per request, for the
+    temperature row and for the heat-flux field (the three flux rows
+    together): the relative difference ||served - FP64|| / ||FP64||, and the
+    difference in standard errors, ||served - FP64|| / ||SE||.  Diffuse walls
+    decorrelate FP32 and FP64 flight after a few bounces, so on such
+    families the difference is the solve's own noise (about one SE)."""
     errs = []
     for i in ids:
         fam, mat, dom, prob = reqs[i]
-        p64 = scenes.problem(prob.nemit, tally=prob.tally, maxscat=prob.maxscat, maxloop=prob.maxloop,
-                             seed=prob.seed, precision=abi.FP64, size=prob.size)
+        n = int(prob.nemit)
         with engine.Scene(mat, dom) as sc:
-            f64 = sc.solve(p64)[0]
-        e = _rownorm_rel(results[i], f64)
-        errs.append({"request": int(i + 1), "family": fam, "bundles": int(prob.nemit),
-                     "temperature": e[0], "heat_flux": max(e[1:])})
+            nemit = sc.plan(prob).nemit
+            cuts = np.linspace(0, nemit, RERUN_PARTS + 1).astype(np.int64)
+            parts = []
+            for a, b in zip(cuts[:-1], cuts[1:]):
+                if b <= a:
+                    continue
+                p64 = scenes.problem(prob.nemit, tally=prob.tally, maxscat=prob.maxscat, maxloop=prob.maxloop,
+                                     seed=prob.seed, precision=abi.FP64, size=prob.size, first=int(a),
+                                     count=int(b - a))
+                parts.append(sc.solve(p64)[0])
+        P = np.array(parts)
+        f64 = P.sum(axis=0)
+        se = np.sqrt(len(P)) * P.std(axis=0, ddof=1) if len(P) > 1 else np.zeros_like(f64)
+        served = np.asarray(results[i])
+
+        def rel(rows):
+            d = np.linalg.norm(served[rows] - f64[rows])
+            nb = np.linalg.norm(f64[rows])
+            ns = np.linalg.norm(se[rows])
+            return float(d / nb) if nb > 0 else float(d), float(d / ns) if ns > 0 else 0.0
+
+        t_rel, t_z = rel(slice(0, 1))
+        q_rel, q_z = rel(slice(1, 4))
+        errs.append({"request": int(i + 1), "family": fam, "bundles": n,
+                     "temperature": t_rel, "heat_flux": q_rel,
+                     "temperature_in_se": t_z, "heat_flux_in_se": q_z})
     return errs
 
 
 def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verbose=False,
-        rerun_fraction=0.0):
+        rerun_fraction=0.0, keep_sampled=False):
     """Serve nreq Poisson arrivals at rate lam; returns a metrics dict.  With
     rerun_fraction > 0, that fraction of the requests (at least one) is
-    solved again in FP64 afterwards and the tally errors are reported."""
+    solved again in FP64 afterwards and the tally errors are reported; with
+    keep_sampled the sampled requests and their served results are returned
+    too, under "_sampled", as (family, material, domain, problem, result)."""
     factory = factory or RequestFactory()
     own = server is None
     server = server or engine.Server(step_bundles=step_bundles)
@@ -152,6 +185,8 @@ def run(lam, nreq, server=None, factory=None, seed=0, step_bundles=1 << 24, verb
     }
     if sampled:
         out["fp64_rerun_errors"] = rerun_errors(reqs, kept, sorted(sampled))
+        if keep_sampled:
+            out["_sampled"] = [reqs[i] + (kept[i],) for i in sorted(sampled)]
     if verbose:
         print(json.dumps(out), flush=True)
     return out

@@ -98,4 +98,7 @@ def test_rerun_sampling_reports_errors():
     errs = out["fp64_rerun_errors"]
     assert len(errs) == 2
     for e in errs:
-        assert 0 <= e["heat_flux"] < 0.5 and 0 <= e["temperature"] < 0.5
+        # FP32 flight and its FP64 rerun share the streams: their difference
+        # is at most the solve's own noise, a few standard errors
+        assert 0 <= e["heat_flux"] < 0.2 and 0 <= e["temperature"] < 0.2
+        assert e["heat_flux_in_se"] < 4 and e["temperature_in_se"] < 4, e
