@@ -94,6 +94,11 @@ constexpr int kSlotQuorum = MCG_SLOT_QUORUM;
 #define MCG_WALK_QUORUM 20
 #endif
 constexpr int kWalkQuorum = MCG_WALK_QUORUM;
+// crossings a walking lane advances between checks of the quorum
+#ifndef MCG_WALK_UNROLL
+#define MCG_WALK_UNROLL 4
+#endif
+constexpr int kWalkUnroll = MCG_WALK_UNROLL;
 
 enum Counter : int { C_NEXT = 0, C_STEPS, C_ESC, C_ABS, C_TRUNC, C_BUNDLES, C_COUNT };
 
@@ -429,16 +434,17 @@ __device__ __forceinline__ bool walk_next(WalkN<NA>& W, const Sink& sink) {
     }
     if (done) return true;
     W.prev = tm;
+    // the first axis at tm advances; selects, not branches, so a crossing
+    // costs no exec-mask changes
     bool open = true;   // no axis advanced yet
 #pragma unroll
     for (int q = 0; q < NA; ++q) {
         const bool take = open && W.t[q] == tm;
         open = open && !take;
-        if (take) {
-            W.idx += W.dcol[q];
-            W.n[q] -= 1;
-            W.t[q] = W.n[q] > 0 ? W.t[q] + W.dt[q] : 2.0f;
-        }
+        const int32_t n = W.n[q] - 1;
+        W.idx += take ? W.dcol[q] : 0;
+        W.n[q] = take ? n : W.n[q];
+        W.t[q] = take ? (n > 0 ? W.t[q] + W.dt[q] : 2.0f) : W.t[q];
     }
     return false;
 }
@@ -1156,6 +1162,13 @@ __device__ __forceinline__ bool emit_now(bool need, bool busy) {
 // next iteration's phase).  With no step to take it runs every walk to its
 // end, so the wavefront's loop always progresses (lanes waiting to emit do
 // not count: emission waits for the walkers).
+// U crossings of the walking lanes (a walk that ends stops there)
+template <int U, int NA>
+__device__ __forceinline__ void walk_some(WalkN<NA>& W, bool& walking, const Sink& sink) {
+    if (walking && walk_next(W, sink)) walking = false;
+    if constexpr (U > 1) walk_some<U - 1>(W, walking, sink);
+}
+
 template <int NA>
 __device__ __forceinline__ void walk_phase(WalkN<NA>& W, bool& walking, bool has_work, const Sink& sink) {
     for (;;) {
@@ -1163,7 +1176,7 @@ __device__ __forceinline__ void walk_phase(WalkN<NA>& W, bool& walking, bool has
         if (m == 0ull) break;
         // lanes with a step to take once their walk is over
         if (popc(m) < kWalkQuorum && __ballot(!walking && has_work) != 0ull) break;
-        if (walking && walk_next(W, sink)) walking = false;
+        walk_some<kWalkUnroll>(W, walking, sink);
     }
 }
 
