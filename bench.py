@@ -496,8 +496,27 @@ def bench_serving(lams=(10, 25, 50, 100, 200, 500), nreq=100, cpu=True, cpu_cap=
     rerun is the first plus the second."""
     from montecarlocpp_amd import serve
     fac = serve.RequestFactory()
+    small = serve.RequestFactory(nmax=1e6)
     sweep = []
     checks = []
+
+    def same_requests(factory, n):
+        """The requests of one stream (the ids of every stream are 1..n)
+        as one mcg_batch_solve, and one mcg_solve each in turn (the plain
+        single-solve path): steps/s of each."""
+        reqs = [factory.make(r + 1) for r in range(n)]
+        scs = [engine.Scene(m, d) for _, m, d, _ in reqs]
+        probs = [p for _, _, _, p in reqs]
+        t0 = time.perf_counter()
+        _, bst = engine.batch_solve(scs, probs)
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        single_steps = sum(sc.solve(p)[1]["steps"] for sc, p in zip(scs, probs))
+        ts = time.perf_counter() - t0
+        for sc in scs:
+            sc.close()
+        return sum(x["steps"] for x in bst) / tb, single_steps / ts
+
     with engine.Server() as sv:
         for lam in lams:
             r = serve.run(lam, nreq, server=sv, factory=fac, seed=int(lam), rerun_fraction=0.01,
@@ -508,23 +527,26 @@ def bench_serving(lams=(10, 25, 50, 100, 200, 500), nreq=100, cpu=True, cpu_cap=
                      f"p50 {r['latency_p50_s'] * 1e3:.1f} ms, p99 {r['latency_p99_s']:.2f} s")
             for (fam, mat, dom, prob, served), err in zip(sampled, r["fp64_rerun_errors"]):
                 checks.append((lam, fam, mat, dom, prob, err))
-    # the batch kernel on the same mix: the requests of one stream (the ids of
-    # every stream are 1..nreq) as one mcg_batch_solve
-    reqs = [fac.make(r + 1) for r in range(nreq)]
-    scs = [engine.Scene(m, d) for _, m, d, _ in reqs]
-    probs = [p for _, _, _, p in reqs]
-    t0 = time.perf_counter()
-    _, bst = engine.batch_solve(scs, probs)
-    tb = time.perf_counter() - t0
-    for sc in scs:
-        sc.close()
-    mix_rate = sum(x["steps"] for x in bst) / tb
+        # a saturating stream of small requests only (1e4-1e6 bundles, all
+        # arriving within a millisecond), where one solve cannot fill the GPU
+        rs = serve.run(1e6, 2 * nreq, server=sv, factory=small, seed=7)
+    mix_rate, single_rate = same_requests(fac, nreq)
+    small_batch, small_single = same_requests(small, 2 * nreq)
     saturated = max(r["scatter_steps_per_s"] for r in sweep)
     out = {"requests_mix": "slab / bar / random slab / porous film, bundles log-uniform 1e4-1e8, "
                            "T_ref 100-500 K, FP32 flight; one server, streams run back to back",
            "requests_per_lambda": nreq, "sweep": sweep,
            "same_requests_one_batch_steps_per_s": mix_rate,
-           "saturated_served_vs_batch": saturated / mix_rate}
+           "same_requests_single_solves_steps_per_s": single_rate,
+           "saturated_served_vs_batch": saturated / mix_rate,
+           "saturated_served_vs_single_solve_path": saturated / single_rate,
+           "small_requests": {"mix": "the same families, bundles log-uniform 1e4-1e6, all arriving at once",
+                              "served": rs, "same_requests_one_batch_steps_per_s": small_batch,
+                              "same_requests_single_solves_steps_per_s": small_single,
+                              "served_vs_single_solve_path": rs["scatter_steps_per_s"] / small_single,
+                              "batch_vs_single_solve_path": small_batch / small_single}}
+    progress(f"serving small requests: {rs['scatter_steps_per_s']:.3g} steps/s served, "
+             f"{small_single:.3g} one solve at a time")
     if cpu:
         from oracle import oracle  # checker leg, after the timed streams
         rows = []

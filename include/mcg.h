@@ -358,7 +358,16 @@ int mcg_batch_destroy(mcg_batch* batch);
  * steps.  Results equal mcg_solve's for the same problem, alone or mixed with
  * other requests.  Steps are pipelined: mcg_server_step enqueues the next
  * step before it waits for the previous one, so a request retires one call
- * after its last bundles were handed out.  One host thread drives a server. */
+ * after its last bundles were handed out.  One host thread drives a server.
+ * mcg_server_submit builds the request's domain, checks that a lower bound
+ * of its device bytes fits the budget, copies the material's arrays and the
+ * problem, and returns; the server's own host threads build the material
+ * tables, the problem and the request's packed device image meanwhile.
+ * mcg_server_step waits for a request's build when it reaches the head of
+ * the queue and admits it with one device allocation and one copy.  A
+ * request whose material or problem is invalid, or that could never fit the
+ * budget, is listed by mcg_server_completed and its error is returned by
+ * mcg_server_poll. */
 typedef struct mcg_server mcg_server;
 
 typedef struct mcg_server_stats {
@@ -377,7 +386,8 @@ int mcg_server_step(mcg_server* server, int32_t* active, int32_t* queued, int32_
 /* Ids of the requests the last mcg_server_step retired (n = how many). */
 int mcg_server_completed(const mcg_server* server, int64_t* ids, int64_t cap, int64_t* n);
 /* MCG_OK and the result (then the request is forgotten), MCG_EAGAIN while it
- * runs, MCG_ENOTFOUND for an unknown id.  stats.wall_ms = submit to done. */
+ * runs, MCG_ENOTFOUND for an unknown id, or the error of a request that
+ * failed (then it is forgotten too).  stats.wall_ms = submit to done. */
 int mcg_server_poll(mcg_server* server, int64_t request_id, double* out, int64_t out_len,
                     mcg_stats* stats);
 int mcg_server_cancel(mcg_server* server, int64_t request_id);

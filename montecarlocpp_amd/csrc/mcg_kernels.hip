@@ -17,12 +17,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1775,10 +1777,19 @@ template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool own = true;   // false: a view into memory owned elsewhere
+    ~DevBuf() { if (p && own) (void)hipFree(p); }
+    // `count` elements at `ptr`, owned by someone else
+    void view(T* ptr, size_t count) {
+        if (p && own) (void)hipFree(p);
+        p = ptr;
+        n = count;
+        own = false;
+    }
     int alloc(size_t count) {
-        if (p) (void)hipFree(p);
+        if (p && own) (void)hipFree(p);
         p = nullptr;
+        own = true;
         n = count;
         if (count == 0) return MCG_OK;
         hipError_t e = hipMalloc(&p, count * sizeof(T));
@@ -2703,6 +2714,111 @@ size_t scene_bytes(const HostScene& h) {
            h.pores.size() * (sizeof(DevPore<double>) + sizeof(DevPore<float>)) +
            modes * (2 * 8 + 2 * 4 + 2 * 8) + h.inter_list.size() * 4;
 }
+
+// One device block holding everything a served request needs: the scene in
+// both precisions, the alias tables, the Inter list and the emitter counts
+// (uploaded from a packed host image in one copy), then the tallies and
+// counters (cleared on the device), contiguous so that one copy reads them
+// back.  Offsets in bytes, each buffer 256-byte aligned.
+struct ImageLayout {
+    size_t planes64, sdoms64, emit64, pores64, vel64, mfp64;
+    size_t planes32, sdoms32, emit32, pores32, vel32, mfp32;
+    size_t scat, flux, inter, cdf;
+    size_t upload;   // bytes of the image copied to the device
+    size_t direct, nd, nf;   // direct (nd doubles), then diff (nf doubles), then the counters
+    size_t total;
+};
+
+namespace {
+template <typename T>
+void put(std::vector<unsigned char>& img, size_t off, const T* src, size_t count) {
+    if (count) std::memcpy(img.data() + off, src, count * sizeof(T));
+}
+
+template <typename F>
+void pack_precision(const HostScene& h, std::vector<unsigned char>& img, size_t planes, size_t sdoms,
+                    size_t emit, size_t pores, size_t vel, size_t mfp) {
+    const double eps_scale = sizeof(F) == 4 ? double(FLT_EPSILON) / DBL_EPSILON : 1.0;
+    std::vector<DevPlane<F>> pl(h.planes.size());
+    for (size_t i = 0; i < pl.size(); ++i) convert_plane(h.planes[i], pl[i]);
+    std::vector<DevSdom<F>> sd(h.sdoms.size());
+    for (size_t i = 0; i < sd.size(); ++i) convert_sdom(h.sdoms[i], sd[i], eps_scale);
+    std::vector<DevEmitter<F>> em(h.emitters.size());
+    for (size_t i = 0; i < em.size(); ++i) convert_emitter(h.emitters[i], em[i]);
+    std::vector<DevPore<F>> po(h.pores.size());
+    for (size_t i = 0; i < po.size(); ++i) convert_pore(h.pores[i], po[i], eps_scale);
+    std::vector<F> mv(h.mode_vel.size()), mm(h.mode_mfp.size());
+    for (size_t i = 0; i < mv.size(); ++i) { mv[i] = F(h.mode_vel[i]); mm[i] = F(h.mode_mfp[i]); }
+    put(img, planes, pl.data(), pl.size());
+    put(img, sdoms, sd.data(), sd.size());
+    put(img, emit, em.data(), em.size());
+    put(img, pores, po.data(), po.size());
+    put(img, vel, mv.data(), mv.size());
+    put(img, mfp, mm.data(), mm.size());
+}
+}  // namespace
+
+// Lays out and packs a request's device image (host work only).
+void pack_request(const HostScene& h, const HostProblem& P, ImageLayout& L, std::vector<unsigned char>& img) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t np = h.planes.size(), ns = h.sdoms.size(), ne = h.emitters.size(), npo = h.pores.size();
+    const size_t nm = h.mode_vel.size();
+    L.planes64 = take(np * sizeof(DevPlane<double>));
+    L.sdoms64 = take(ns * sizeof(DevSdom<double>));
+    L.emit64 = take(ne * sizeof(DevEmitter<double>));
+    L.pores64 = take(npo * sizeof(DevPore<double>));
+    L.vel64 = take(nm * sizeof(double));
+    L.mfp64 = take(nm * sizeof(double));
+    L.planes32 = take(np * sizeof(DevPlane<float>));
+    L.sdoms32 = take(ns * sizeof(DevSdom<float>));
+    L.emit32 = take(ne * sizeof(DevEmitter<float>));
+    L.pores32 = take(npo * sizeof(DevPore<float>));
+    L.vel32 = take(nm * sizeof(float));
+    L.mfp32 = take(nm * sizeof(float));
+    L.scat = take(h.scat_alias.size() * sizeof(uint64_t));
+    L.flux = take(h.flux_alias.size() * sizeof(uint64_t));
+    L.inter = take(h.inter_list.size() * sizeof(int32_t));
+    L.cdf = take(P.emit_cdf.size() * sizeof(int64_t));
+    L.upload = off;
+    L.nd = size_t(P.rows) * size_t(h.cols);
+    L.nf = size_t(P.rows) * size_t(h.cols + 1);
+    L.direct = take((L.nd + L.nf + C_COUNT) * sizeof(double));
+    L.total = off;
+    img.assign(L.upload, 0);
+    pack_precision<double>(h, img, L.planes64, L.sdoms64, L.emit64, L.pores64, L.vel64, L.mfp64);
+    pack_precision<float>(h, img, L.planes32, L.sdoms32, L.emit32, L.pores32, L.vel32, L.mfp32);
+    put(img, L.scat, h.scat_alias.data(), h.scat_alias.size());
+    put(img, L.flux, h.flux_alias.data(), h.flux_alias.size());
+    put(img, L.inter, h.inter_list.data(), h.inter_list.size());
+    put(img, L.cdf, P.emit_cdf.data(), P.emit_cdf.size());
+}
+
+// Points the scene's device buffers into a request's block.
+void view_scene(mcg_scene* sc, unsigned char* b, const ImageLayout& L) {
+    const HostScene& h = sc->host;
+    const size_t np = h.planes.size(), ns = h.sdoms.size(), ne = h.emitters.size(), npo = h.pores.size();
+    const size_t nm = h.mode_vel.size();
+    sc->d64.planes.view(reinterpret_cast<DevPlane<double>*>(b + L.planes64), np);
+    sc->d64.sdoms.view(reinterpret_cast<DevSdom<double>*>(b + L.sdoms64), ns);
+    sc->d64.emitters.view(reinterpret_cast<DevEmitter<double>*>(b + L.emit64), ne);
+    sc->d64.pores.view(reinterpret_cast<DevPore<double>*>(b + L.pores64), npo);
+    sc->d64.mode_vel.view(reinterpret_cast<double*>(b + L.vel64), nm);
+    sc->d64.mode_mfp.view(reinterpret_cast<double*>(b + L.mfp64), nm);
+    sc->d32.planes.view(reinterpret_cast<DevPlane<float>*>(b + L.planes32), np);
+    sc->d32.sdoms.view(reinterpret_cast<DevSdom<float>*>(b + L.sdoms32), ns);
+    sc->d32.emitters.view(reinterpret_cast<DevEmitter<float>*>(b + L.emit32), ne);
+    sc->d32.pores.view(reinterpret_cast<DevPore<float>*>(b + L.pores32), npo);
+    sc->d32.mode_vel.view(reinterpret_cast<float*>(b + L.vel32), nm);
+    sc->d32.mode_mfp.view(reinterpret_cast<float*>(b + L.mfp32), nm);
+    sc->scat_alias.view(reinterpret_cast<uint64_t*>(b + L.scat), h.scat_alias.size());
+    sc->flux_alias.view(reinterpret_cast<uint64_t*>(b + L.flux), h.flux_alias.size());
+    sc->inter_list.view(reinterpret_cast<int32_t*>(b + L.inter), h.inter_list.size());
+}
 }  // namespace mcg
 
 
@@ -2860,14 +2976,25 @@ struct Request {
     HostProblem P;
     int64_t cursor = 0;     // bundles handed out (offset from P.first)
     int64_t done = 0;       // bundles of completed steps
-    size_t bytes = 0;       // device memory held while admitted
+    size_t bytes = 0;       // device memory held while admitted (the block)
     DevBuf<double> direct, diff;
     DevBuf<unsigned long long> ctr;
     DevBuf<int64_t> cdf;
-    int state = 0;          // 0 queued, 1 active, 2 done
+    int state = 0;          // 0 queued, 1 active, 2 done, 3 failed
     std::vector<double> result;
     mcg_stats stats{};
     std::chrono::steady_clock::time_point t_submit, t_done;
+    // built by a worker thread after submission: the material (from copies
+    // of the spec's arrays), the problem and the packed device image
+    std::vector<double> disp, relax;
+    mcg_material_spec mspec{};
+    mcg_problem_spec pspec{};
+    ImageLayout lay{};
+    std::vector<unsigned char> image;
+    int built = 0;          // 0 building, 1 built, -1 failed (guarded by the server's mutex)
+    int err_code = 0;
+    std::string err_msg;
+    void* block = nullptr;  // the device block, while admitted and until buried
 };
 
 // One enqueued step: its launches' buffers, timing events and hand-outs.
@@ -2896,6 +3023,14 @@ struct StepSlot {
 
 struct mcg_server {
     size_t budget = 0, used = 0;
+    int device = 0, num_cu = 0;
+    size_t lds_limit = 0;
+    // host threads building submitted requests (material, problem, image)
+    std::mutex mu;
+    std::condition_variable cv_todo, cv_built;
+    std::deque<mcg::Request*> todo;
+    std::vector<std::thread> workers;
+    bool stopping = false;
     int64_t step_bundles = 0;
     mcg_tune tune{};
     int64_t next_id = 1;
@@ -2914,15 +3049,77 @@ struct mcg_server {
 namespace mcg {
 namespace {
 
-int admit(mcg_server* sv, Request* r) {
-    int rc = scene_upload(r->scene.get());
+// Host work of a submitted request, on a worker thread: the material tables,
+// the problem and the packed device image (no HIP call).
+int build_request(Request* r) {
+    int rc = build_material(&r->mspec, r->scene->host);
+    if (!rc) rc = build_problem(&r->pspec, r->scene->host, r->P);
     if (rc) return rc;
-    const size_t nd = size_t(r->P.rows) * r->scene->host.cols;
-    const size_t nf = size_t(r->P.rows) * (r->scene->host.cols + 1);
-    if ((rc = r->direct.alloc(nd))) return rc;
-    if ((rc = r->diff.alloc(nf))) return rc;
-    if ((rc = r->ctr.alloc(C_COUNT))) return rc;
-    if ((rc = r->cdf.upload(r->P.emit_cdf.data(), r->P.emit_cdf.size()))) return rc;
+    pack_request(r->scene->host, r->P, r->lay, r->image);
+    r->bytes = r->lay.total;
+    return MCG_OK;
+}
+
+void build_worker(mcg_server* sv) {
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv_todo.wait(lk, [&] { return sv->stopping || !sv->todo.empty(); });
+            if (sv->stopping) return;   // the server is going away: what is left is not built
+            r = sv->todo.front();
+            sv->todo.pop_front();
+        }
+        const int rc = build_request(r);
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            r->built = rc ? -1 : 1;
+            if (rc) {
+                r->err_code = rc;
+                r->err_msg = t_err;
+            }
+        }
+        sv->cv_built.notify_all();
+    }
+}
+
+// Blocks until the request's build has finished (or failed).
+void wait_built(mcg_server* sv, Request* r) {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_built.wait(lk, [&] { return r->built != 0; });
+}
+
+void stop_workers(mcg_server* sv) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stopping = true;
+    }
+    sv->cv_todo.notify_all();
+    for (std::thread& t : sv->workers)
+        if (t.joinable()) t.join();
+    sv->workers.clear();
+}
+
+// A request's device side: one block, its image uploaded in one copy.
+int admit(mcg_server* sv, Request* r) {
+    const ImageLayout& L = r->lay;
+    void* b = nullptr;
+    const hipError_t e = hipMalloc(&b, L.total);
+    if (e != hipSuccess) return set_error(MCG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    r->block = b;
+    if (L.upload) HIP_TRY(hipMemcpy(b, r->image.data(), L.upload, hipMemcpyHostToDevice));
+    std::vector<unsigned char>().swap(r->image);
+    unsigned char* const base = static_cast<unsigned char*>(b);
+    mcg_scene* sc = r->scene.get();
+    sc->device = sv->device;
+    sc->num_cu = sv->num_cu;
+    sc->lds_limit = sv->lds_limit;
+    view_scene(sc, base, L);
+    double* const direct = reinterpret_cast<double*>(base + L.direct);
+    r->direct.view(direct, L.nd);
+    r->diff.view(direct + L.nd, L.nf);
+    r->ctr.view(reinterpret_cast<unsigned long long*>(direct + L.nd + L.nf), C_COUNT);
+    r->cdf.view(reinterpret_cast<int64_t*>(base + L.cdf), r->P.emit_cdf.size());
     // cleared on the stream of the first step that uses the request
     sv->fresh.push_back(r);
     r->state = 1;
@@ -2934,23 +3131,22 @@ int admit(mcg_server* sv, Request* r) {
 // Free device memory of retired requests (called when nothing is in flight).
 void bury(mcg_server* sv) {
     for (auto& r : sv->graveyard) {
-        r->direct.alloc(0);
-        r->diff.alloc(0);
-        r->ctr.alloc(0);
-        r->cdf.alloc(0);
         r->scene.reset();
+        if (r->block) (void)hipFree(r->block);
+        r->block = nullptr;
     }
     sv->graveyard.clear();
 }
 
 int retire(mcg_server* sv, Request* r) {
-    std::vector<double> hd(r->direct.n), hf(r->diff.n);
+    // tallies and counters are contiguous in the block: one copy
+    const size_t nd = r->direct.n, nf = r->diff.n;
+    std::vector<double> h(nd + nf + C_COUNT);
+    HIP_TRY(hipMemcpy(h.data(), r->direct.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     unsigned long long hc[C_COUNT];
-    HIP_TRY(hipMemcpy(hd.data(), r->direct.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hf.data(), r->diff.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hc, r->ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
+    std::memcpy(hc, h.data() + nd + nf, sizeof(hc));
     r->result.assign(size_t(r->P.rows * r->scene->host.cols), 0.0);
-    finalise_field(r->scene->host, r->P, hd.data(), hf.data(), r->result.data());
+    finalise_field(r->scene->host, r->P, h.data(), h.data() + nd, r->result.data());
     r->stats.bundles = int64_t(hc[C_BUNDLES]);
     r->stats.steps = int64_t(hc[C_STEPS]);
     r->stats.escapes = int64_t(hc[C_ESC]);
@@ -3566,7 +3762,20 @@ int mcg_server_create(uint64_t hbm_budget_bytes, int64_t step_bundles, const mcg
     sv->budget = hbm_budget_bytes > 0 ? size_t(hbm_budget_bytes) : size_t(200) << 30;
     sv->step_bundles = step_bundles > 0 ? step_bundles : (int64_t(1) << 24);
     if (tune) sv->tune = *tune;
+    HIP_TRY(hipGetDevice(&sv->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, sv->device));
+    sv->num_cu = prop.multiProcessorCount;
+    sv->lds_limit = prop.sharedMemPerBlock;
     for (mcg::StepSlot& sl : sv->slot) HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nworkers = int(std::max(1u, std::min(8u, hc > 1 ? hc / 2 : 1u)));
+    try {
+        for (int i = 0; i < nworkers; ++i) sv->workers.emplace_back(mcg::build_worker, sv.get());
+    } catch (const std::exception&) {
+        mcg::stop_workers(sv.get());
+        return mcg::set_error(MCG_ENOMEM, "cannot start the server's build threads");
+    }
     *out = sv.release();
     return MCG_OK;
 }
@@ -3578,19 +3787,43 @@ int mcg_server_submit(mcg_server* sv, const mcg_material_spec* mat, const mcg_do
     if (!r) return mcg::set_error(MCG_ENOMEM, "out of host memory");
     r->scene.reset(new (std::nothrow) mcg_scene());
     if (!r->scene) return mcg::set_error(MCG_ENOMEM, "out of host memory");
-    int rc = mcg::build_material(mat, r->scene->host);
-    if (!rc) rc = mcg::build_domain(dom, r->scene->host);
-    if (!rc) rc = mcg::build_problem(prob, r->scene->host, r->P);
+    if (!mat || !prob) return mcg::set_error(MCG_EINVAL, "NULL argument");
+    // what the copies below rely on; the rest of the material is checked by its build
+    if (!mat->disp || !mat->relax) return mcg::set_error(MCG_EINVAL, "material spec is NULL");
+    if (mat->nw <= 0 || mat->np <= 0) return mcg::set_error(MCG_EINVAL, "Invalid dispersion file");
+    // the domain is built now (its spec's arrays are the caller's); the
+    // material, problem and device image on a build thread, from copies
+    int rc = mcg::build_domain(dom, r->scene->host);
     if (rc) return rc;
-    r->bytes = mcg::scene_bytes(r->scene->host) +
-               sizeof(double) * size_t(r->P.rows) * (2 * r->scene->host.cols + 1) +
-               sizeof(unsigned long long) * mcg::C_COUNT + sizeof(int64_t) * r->P.emit_cdf.size();
-    if (r->bytes > sv->budget) return mcg::set_error(MCG_ENOMEM, "request exceeds the device memory budget");
+    const mcg::HostScene& h = r->scene->host;
+    {
+        // a lower bound of the request's device bytes: geometry and tallies
+        const int64_t rows = prob->tally == MCG_TEMP ? 1 : prob->tally == MCG_FLUX ? 3 : prob->tally == MCG_MULTI ? 4
+                             : prob->tally == MCG_CUMTEMP ? std::max<int64_t>(1, prob->size + 1)
+                                                          : 3 * std::max<int64_t>(1, prob->size + 1);
+        const size_t lower = h.planes.size() * (sizeof(mcg::DevPlane<double>) + sizeof(mcg::DevPlane<float>)) +
+                             h.sdoms.size() * (sizeof(mcg::DevSdom<double>) + sizeof(mcg::DevSdom<float>)) +
+                             sizeof(double) * size_t(rows) * (2 * size_t(h.cols) + 1);
+        if (lower > sv->budget) return mcg::set_error(MCG_ENOMEM, "request exceeds the device memory budget");
+    }
+    const size_t nd = size_t(mat->nw) * size_t(2 + 2 * mat->np), nr = size_t(mat->np) * 8;
+    r->disp.assign(mat->disp, mat->disp + nd);
+    r->relax.assign(mat->relax, mat->relax + nr);
+    r->mspec = *mat;
+    r->mspec.disp = r->disp.data();
+    r->mspec.relax = r->relax.data();
+    r->pspec = *prob;
     r->id = sv->next_id++;
     r->t_submit = std::chrono::steady_clock::now();
     *request_id = r->id;
-    sv->queue.push_back(r.get());
+    mcg::Request* raw = r.get();
+    sv->queue.push_back(raw);
     sv->all[r->id] = std::move(r);
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->todo.push_back(raw);
+    }
+    sv->cv_todo.notify_one();
     return MCG_OK;
 }
 
@@ -3598,9 +3831,24 @@ int mcg_server_step(mcg_server* sv, int32_t* active, int32_t* queued, int32_t* c
     if (!sv) return mcg::set_error(MCG_EINVAL, "NULL server");
     int rc;
     sv->completed.clear();
-    // admission in arrival order while the budget allows
-    while (!sv->queue.empty() && sv->used + sv->queue.front()->bytes <= sv->budget) {
+    // admission in arrival order while the budget allows (a request's build
+    // is waited for when it reaches the head of the queue; one that failed,
+    // or that could never fit, is completed with its error)
+    while (!sv->queue.empty()) {
         mcg::Request* r = sv->queue.front();
+        mcg::wait_built(sv, r);
+        if (r->built > 0 && r->bytes > sv->budget) {
+            r->built = -1;
+            r->err_code = MCG_ENOMEM;
+            r->err_msg = "request exceeds the device memory budget";
+        }
+        if (r->built < 0) {
+            sv->queue.pop_front();
+            r->state = 3;
+            sv->completed.push_back(r->id);
+            continue;
+        }
+        if (sv->used + r->bytes > sv->budget) break;
         sv->queue.pop_front();
         if ((rc = mcg::admit(sv, r))) return rc;
     }
@@ -3636,6 +3884,13 @@ int mcg_server_poll(mcg_server* sv, int64_t id, double* out, int64_t out_len, mc
     auto it = sv->all.find(id);
     if (it == sv->all.end()) return mcg::set_error(MCG_ENOTFOUND, "unknown request id");
     mcg::Request* r = it->second.get();
+    if (r->state == 3) {
+        const int code = r->err_code;
+        const std::string msg = r->err_msg;
+        sv->graveyard.push_back(std::move(it->second));
+        sv->all.erase(it);
+        return mcg::set_error(code, msg);
+    }
     if (r->state != 2) return MCG_EAGAIN;
     if (out) {
         if (out_len < int64_t(r->result.size())) return mcg::set_error(MCG_ERANGE, "output buffer too small");
@@ -3653,6 +3908,8 @@ int mcg_server_cancel(mcg_server* sv, int64_t id) {
     if (it == sv->all.end()) return mcg::set_error(MCG_ENOTFOUND, "unknown request id");
     mcg::Request* r = it->second.get();
     if (r->state == 0) {
+        // a build thread may hold it: wait for the build before letting go
+        mcg::wait_built(sv, r);
         sv->queue.erase(std::remove(sv->queue.begin(), sv->queue.end(), r), sv->queue.end());
     } else if (r->state == 1) {
         // the request may still have work in flight: wait for it first
@@ -3682,8 +3939,11 @@ int mcg_server_info(const mcg_server* sv, mcg_server_stats* out) {
 
 int mcg_server_destroy(mcg_server* sv) {
     if (!sv) return MCG_OK;
+    mcg::stop_workers(sv);   // builds in progress finish first
     for (mcg::StepSlot& sl : sv->slot)
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    for (auto& kv : sv->all) sv->graveyard.push_back(std::move(kv.second));
+    sv->all.clear();
     mcg::bury(sv);
     delete sv;
     return MCG_OK;

@@ -296,6 +296,12 @@ def autotune_batch(scenes_, probs, sample_fraction=0.1):
     return t, r.value
 
 
+def tally_rows(tally, size):
+    """Rows of a problem's output (TempProblem 1, FluxProblem 3, MultiProblem
+    4, CumTempProblem size + 1, CumFluxProblem 3 (size + 1))."""
+    return {abi.TEMP: 1, abi.FLUX: 3, abi.MULTI: 4, abi.CUMTEMP: size + 1}.get(tally, 3 * (size + 1))
+
+
 class Server:
     """Continuous-batching request server (mcg_server_*)."""
 
@@ -324,11 +330,15 @@ class Server:
         self.close()
 
     def submit(self, mat, dom, prob):
-        info, _, _ = plan(mat, dom, prob)
+        # the output shape without building the material: rows from the
+        # tally kind, columns from the domain alone
+        ds = dom.spec()
+        cols = C.c_int64()
+        check(lib().mcg_cell_volumes(C.byref(ds), None, 0, C.byref(cols)))
+        rows = tally_rows(prob.tally, prob.size)
         rid = C.c_int64()
-        check(lib().mcg_server_submit(self.handle, C.byref(mat.spec), C.byref(dom.spec()),
-                                      C.byref(prob), C.byref(rid)))
-        self._keep[rid.value] = (mat, dom, info.rows, info.cols)
+        check(lib().mcg_server_submit(self.handle, C.byref(mat.spec), C.byref(ds), C.byref(prob), C.byref(rid)))
+        self._keep[rid.value] = (mat, dom, rows, cols.value)
         return rid.value
 
     def step(self):
@@ -353,8 +363,8 @@ class Server:
         rc = lib().mcg_server_poll(self.handle, rid, abi.dptr(out), out.size, C.byref(st))
         if rc == abi.MCG_EAGAIN:
             return None
+        del self._keep[rid]    # done or failed: the server has forgotten it
         check(rc)
-        del self._keep[rid]
         return out.reshape(cols, rows).T.copy(), st.as_dict()
 
     def cancel(self, rid):

@@ -102,3 +102,39 @@ def test_rerun_sampling_reports_errors():
         # is at most the solve's own noise, a few standard errors
         assert 0 <= e["heat_flux"] < 0.2 and 0 <= e["temperature"] < 0.2
         assert e["heat_flux_in_se"] < 4 and e["temperature_in_se"] < 4, e
+
+
+def test_failed_request_reported():
+    """A request is built on the server's host threads after mcg_server_submit
+    returns: one whose problem cannot be solved (a closed box with no source)
+    is completed with its error, which poll returns, and the requests around
+    it are served as usual."""
+    fac = serve.RequestFactory(nmin=1e4, nmax=3e4, families=("slab", "bar"))
+    good = [fac.make(r) for r in range(1, 4)]
+    sp = scenes.face(abi.SPEC)
+    closed = scenes.Domain("closed")
+    closed.add_box((0, 0, 0), (1e-7, 1e-7, 1e-7), (0, 0, 0), [sp] * 6)
+    with engine.Server(step_bundles=40_000) as sv:
+        ids = [sv.submit(*good[0][1:]), sv.submit(good[1][1], closed, scenes.problem(1000, seed=2)),
+               sv.submit(*good[2][1:])]
+        done, failed = {}, []
+        for _ in range(100000):
+            sv.step()
+            for rid in sv.completed():
+                if rid == ids[1]:
+                    with pytest.raises(engine.EngineError, match="No phonons to emit"):
+                        sv.poll(rid)
+                    failed.append(rid)
+                else:
+                    done[rid] = sv.poll(rid)
+            if len(done) + len(failed) == 3:
+                break
+        assert failed == [ids[1]] and sorted(done) == sorted([ids[0], ids[2]])
+        with pytest.raises(engine.EngineError) as e:
+            sv.poll(ids[1])
+        assert e.value.code == abi.MCG_ENOTFOUND
+    for rid, r in ((ids[0], good[0]), (ids[2], good[2])):
+        with engine.Scene(r[1], r[2]) as sc:
+            f, st, _ = sc.solve(r[3])
+        assert done[rid][1]["steps"] == st["steps"]
+        assert np.all(rownorm_rel(done[rid][0], f) < 1e-10)
