@@ -775,6 +775,41 @@ struct LaneStats {
     }
 };
 
+// FP32 flight at an Inter face with several partners, when rounding has
+// left the point inside none of them: the partner it is least outside of
+// (the deepest face violation, in units of that cell's tolerance), with the
+// point moved back onto the violated faces (a shift of the order of the
+// rounding), so that the flight continues there instead of the bundle being
+// lost.  -1 if no partner is usable.
+template <typename F>
+__device__ __forceinline__ int inter_nearest(const View<F>& V, const DevPlane<F>& P, const int32_t* list,
+                                          dev::V3<F>& pos) {
+    int best = -1;
+    F best_depth = F(-3.0e38);
+    for (int c = 0; c < P.icnt; ++c) {
+        const int cand = list[P.ibeg + c];
+        const DevSdom<F>& C = V.sdoms[cand];
+        F depth = F(3.0e38);
+        for (int b = C.pbeg; b < C.pend; ++b) {
+            const DevPlane<F>& Q = V.planes[b];
+            depth = fminf(depth, (dev::dot3(Q.n, pos) + Q.c) / C.eps);
+        }
+        if (C.pore < 0 && depth > best_depth) {
+            best_depth = depth;
+            best = cand;
+        }
+    }
+    if (best >= 0) {
+        const DevSdom<F>& C = V.sdoms[best];
+        for (int b = C.pbeg; b < C.pend; ++b) {
+            const DevPlane<F>& Q = V.planes[b];
+            const F d = dev::dot3(Q.n, pos) + Q.c;
+            if (d < F(0)) pos = dev::V3<F>{pos.x - d * Q.n[0], pos.y - d * Q.n[1], pos.z - d * Q.n[2]};
+        }
+    }
+    return best;
+}
+
 // The intrinsic and planar-face scatters of FP32 flight in one code path:
 // the isotropic (material.cpp:226-231) and Lambertian (boundary.cpp:308-312)
 // draws share the event's words and differ only in how the polar angle is
@@ -827,6 +862,7 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
                     break;
                 }
             }
+            if (target < 0) target = inter_nearest(V, P, K.inter_list, L.pos);
         }
         if (target < 0) fate = MCG_ESCAPED;
         else L.sdom = target;
@@ -850,7 +886,12 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     F dmin = L.scat_next;
     int hit = -1;
     // FP32 keeps each face's signed distance and approach rate for the kill
-    // test below (at most kMaxFaces faces per cell, unrolled into registers)
+    // test below (at most kMaxFaces faces per cell, unrolled into registers).
+    // A bundle that rounding left just outside a face it moves towards meets
+    // that face where it stands (parameter 0) instead of a parameter below
+    // zero, which at a grazing angle would exceed the kill tolerance: the
+    // face's rule (reflection, Inter crossing, absorption) applies instead of
+    // a loss.  FP64 keeps the checker's arithmetic.
     F dist[kMaxFaces], rate[kMaxFaces];
     if constexpr (sizeof(F) == 4) {
         if (S.box) {
@@ -860,7 +901,8 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const bool up = u[a] > F(0);
-                const F t = ((up ? S.bmax[a] : S.bmin[a]) - p[a]) * __builtin_amdgcn_rcpf(u[a]);
+                const F ta = ((up ? S.bmax[a] : S.bmin[a]) - p[a]) * __builtin_amdgcn_rcpf(u[a]);
+                const F t = ta < F(0) ? F(0) : ta;
                 if (u[a] != F(0) && t < dmin) {
                     dmin = t;
                     hit = S.pbeg + (up ? S.fmax[a] : S.fmin[a]);
@@ -879,7 +921,8 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
                     const F nd = dev::dot3(P.n, L.dir);
                     const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
                                                                                 __builtin_fmaf(P.n[0], L.pos.x, P.c)));
-                    const F t = -d * __builtin_amdgcn_rcpf(nd);
+                    const F tf = -d * __builtin_amdgcn_rcpf(nd);
+                    const F t = tf < F(0) ? F(0) : tf;
                     if (nd < F(0) && t < dmin) {
                         dmin = t;
                         hit = b;
@@ -895,7 +938,8 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
                 const F nd = dev::dot3(P.n, L.dir);
                 const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
                                                                             __builtin_fmaf(P.n[0], L.pos.x, P.c)));
-                const F t = -d * __builtin_amdgcn_rcpf(nd);
+                const F tf = -d * __builtin_amdgcn_rcpf(nd);
+                const F t = tf < F(0) ? F(0) : tf;
                 if (nd < F(0) && t < dmin) {
                     dmin = t;
                     hit = b;

@@ -3,7 +3,7 @@ tetrahedron with a volume source and a skewed parallelepiped, each with an
 isothermal face and diffuse / specular / partial walls and a 1-D to 3-D grid.
 Deterministic mode equals the FP64 checker bundle by bundle (counts exact,
 tallies to 1e-5); FP32 flight, which advects such cells face by face rather
-than axis by axis, runs every bundle, loses at most one in ten thousand (two in a small solve), and its
+than axis by axis, runs every bundle, loses at most one in a million (one in a small solve), and its
 heat flux agrees with FP64 within 4 standard errors over seeds."""
 import numpy as np
 import pytest
@@ -63,7 +63,7 @@ def test_cell_fp32_flight(si300, name):
             for prec, out in ((abi.FP32_FLIGHT, q32), (abi.FP64, q64)):
                 f, st, _ = sc.solve(scenes.problem(40000, maxscat=60, seed=seed, precision=prec))
                 assert st["bundles"] == 40000
-                assert st["escapes"] <= max(2, st["bundles"] // 10000), st["escapes"]
+                assert st["escapes"] <= max(1, st["bundles"] // 1_000_000), st["escapes"]
                 out.append(f[1].sum())
     q32, q64 = np.array(q32), np.array(q64)
     se = np.sqrt(q32.var(ddof=1) / len(q32) + q64.var(ddof=1) / len(q64))

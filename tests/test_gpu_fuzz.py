@@ -3,7 +3,7 @@ random boxes (face kinds, periodic pairs, volume gradients, pores, Inter
 splits, grids of 0 to 3 dimensions, every tally kind, maxscat), the HIP
 engine's per-bundle counts equal the FP64 checker's exactly and the tallies
 agree to 1e-5, in a single solve and inside a batch; FP32 flight on the same
-scenes runs every bundle and loses at most one in ten thousand (two in a small solve)."""
+scenes runs every bundle and loses at most one in a million (one in a small solve)."""
 import numpy as np
 import pytest
 
@@ -122,8 +122,8 @@ def test_fuzz_batched_fp64_and_fp32(cases):
                 assert sb[k]["steps"] == s1["steps"], k
                 assert np.all(rownorm_rel(fb[k], f1) < 1e-10), k
             else:   # FP32 rounding may lose a bundle on a face, rarely
-                assert sb[k]["escapes"] <= max(2, sb[k]["bundles"] // 10000), (k, sb[k]["escapes"])
-                assert s1["escapes"] <= max(2, s1["bundles"] // 10000), (k, s1["escapes"])
+                assert sb[k]["escapes"] <= max(1, sb[k]["bundles"] // 1_000_000), (k, sb[k]["escapes"])
+                assert s1["escapes"] <= max(1, s1["bundles"] // 1_000_000), (k, s1["escapes"])
             assert np.all(np.isfinite(fb[k]))
         for sc in scs:
             sc.close()

@@ -3,7 +3,7 @@ prism with a volume source, a 4-gon pyramid, a regular 9-gon prism (11
 faces: more than an FP32 step keeps in registers) and the 42-cell octet-truss
 unit (OctetDomain) built from them.  Deterministic mode equals
 the FP64 checker bundle by bundle; FP32 flight runs every bundle, loses at
-most one in ten thousand (two in a small solve) and agrees with FP64 within 4 standard errors over
+most one in a million (one in a small solve) and agrees with FP64 within 4 standard errors over
 seeds.  A rectangular 4-gon prism against the parallelepiped of the same box
 gives the same heat flux within 4 standard errors, once the reference's prism
 volume (half the box's: its volume loop leaves out the fan's last triangle)
@@ -61,7 +61,7 @@ def test_ngon_fp32_flight(si300, name):
                 prob = scenes.problem(40000, maxscat=60, seed=seed, precision=prec)
                 f, st, _ = sc.solve(prob)
                 assert st["bundles"] == engine.plan(si300, d, prob)[0].nemit
-                assert st["escapes"] <= max(2, st["bundles"] // 10000), st["escapes"]
+                assert st["escapes"] <= max(1, st["bundles"] // 1_000_000), st["escapes"]
                 out.append(f[1:4].sum())
     q32, q64 = np.array(q32), np.array(q64)
     se = np.sqrt(q32.var(ddof=1) / len(q32) + q64.var(ddof=1) / len(q64))
@@ -79,7 +79,7 @@ def test_rectangular_prism_is_the_box(si300):
             for sc, out in ((sp, qp), (sb, qb)):
                 f, st, _ = sc.solve(scenes.problem(100000, maxscat=80, seed=seed,
                                                    precision=abi.FP32_FLIGHT))
-                assert st["bundles"] == 100000 and st["escapes"] <= 10
+                assert st["bundles"] == 100000 and st["escapes"] <= 1
                 out.append(f[1, 0])
     qp, qb = np.array(qp) / 2.0, np.array(qb)
     se = np.sqrt(qp.var(ddof=1) / len(qp) + qb.var(ddof=1) / len(qb))

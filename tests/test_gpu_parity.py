@@ -132,7 +132,7 @@ def test_bundle_ids_past_32_bits(si300):
     p32 = scenes.problem(1 << 33, seed=21, first=first, count=6000, precision=abi.FP32_FLIGHT)
     with engine.Scene(si300, dom) as sc:
         _, st, _ = sc.solve(p32)
-    assert st["bundles"] == 6000 and st["escapes"] <= 6
+    assert st["bundles"] == 6000 and st["escapes"] <= 1
 
 
 @pytest.mark.parametrize("name", ["jct", "tee", "tube"])
@@ -246,5 +246,5 @@ def test_fp32_large_solve_every_bundle_once(si300):
     assert np.all(c32.nstep >= 1)
     assert int(c32.nstep.astype(np.int64).sum()) == s32["steps"]
     assert np.all(c32.fate <= 3)
-    assert s32["escapes"] <= n // 1000
+    assert s32["escapes"] <= max(1, n // 1_000_000)
     assert abs(f32[1].mean() / f64[1].mean() - 1) < 1e-3
