@@ -380,6 +380,17 @@ def bench_octet(cpu=True, nbundles=10_000_000, cpu_sample=200_000):
     return out
 
 
+def served_wait(sv, ids):
+    """Steps the server until every id has a result; returns {id: (field, stats)}."""
+    want, res = set(ids), {}
+    while len(res) < len(want):
+        sv.step()
+        for rid in sv.completed():
+            if rid in want:
+                res[rid] = sv.poll(rid)
+    return res
+
+
 def bench_batched(cpu=True):
     """256 independent slabs (random thickness, T_ref, lateral specularity,
     bundle count): one persistent launch with the autotuned batch shape, its
@@ -415,6 +426,17 @@ def bench_batched(cpu=True):
     ts = time.perf_counter() - t0
     for sc in scs:
         sc.close()
+    # the same 256 solves submitted at once to the continuous-batching
+    # server: from the first submission to the last result (material tables
+    # and uploads included)
+    with engine.Server() as sv:
+        warm = [sv.submit(*cases[i]) for i in range(4)]
+        served_wait(sv, warm)
+        t0 = time.perf_counter()
+        ids = [sv.submit(m, d, p) for m, d, p in cases]
+        res = served_wait(sv, ids)
+        tv = time.perf_counter() - t0
+    served_steps = sum(r[1]["steps"] for r in res.values())
     out = {"workload": "256 slabs, L log-uniform 10 nm-10 um, T_ref 100-500 K, dT 2 K, "
                        "lateral specularity U[0,1], 1e4-1e6 bundles each (5.4e7 total), FP32 flight",
            "bundles": int(sum(p.nemit for p in probs)), "scatter_steps": int(steps), "escapes": int(lost),
@@ -426,7 +448,9 @@ def bench_batched(cpu=True):
            "graph_replay_steps_per_s": gsteps / tg, "graph_replay_ms": tg * 1e3,
            "single_solve_path_steps_per_s": steps / ts, "single_solve_path_ms": ts * 1e3,
            "single_solve_path_kernel_ms": single_kernel_ms,
-           "speedup_vs_single_solve_path": ts / tb}
+           "speedup_vs_single_solve_path": ts / tb,
+           "served_steps_per_s": served_steps / tv, "served_ms": tv * 1e3,
+           "served_vs_single_solve_path": (served_steps / tv) / (steps / ts)}
     if cpu:
         c64 = workloads.slab_batch(256, precision=abi.FP64)
         rate, csteps, secs = cpu_median(c64)

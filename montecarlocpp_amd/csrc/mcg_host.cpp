@@ -345,6 +345,17 @@ int build_material(const mcg_material_spec* spec, HostScene& s) {
     // relaxation times: 1/tau = sum over mechanisms (material.cpp:116-134)
     s.tau.assign(n, 0.0);
     std::vector<double> rate(nw);
+    // omega^c1 once per distinct exponent (branches and mechanisms share
+    // them): the same std::pow values, computed fewer times
+    std::vector<std::pair<double, std::vector<double>>> powers;
+    auto omega_pow = [&](double e) -> const std::vector<double>& {
+        for (const auto& q : powers)
+            if (q.first == e) return q.second;
+        std::vector<double> v(nw);
+        for (int64_t w = 0; w < nw; ++w) v[w] = std::pow(omega[w], e);
+        powers.emplace_back(e, std::move(v));
+        return powers.back().second;
+    };
     for (int64_t p = 0; p < np; ++p) {
         std::fill(rate.begin(), rate.end(), 0.0);
         for (int mech = 0; mech < 2; ++mech) {
@@ -352,8 +363,9 @@ int build_material(const mcg_material_spec* spec, HostScene& s) {
             if (c[0] < 0.0) return set_error(MCG_EINVAL, "Scattering times cannot be negative");
             if (c[0] <= DBL_MIN) continue;
             const double tp = std::pow(T, c[2]), ex = std::exp(-c[3] / T);
+            const std::vector<double>& pw = omega_pow(c[1]);
             for (int64_t w = 0; w < nw; ++w)
-                rate[w] = rate[w] + ((c[0] * std::pow(omega[w], c[1])) * tp) * ex;
+                rate[w] = rate[w] + ((c[0] * pw[w]) * tp) * ex;
         }
         for (int64_t w = 0; w < nw; ++w) {
             double t = 1.0 / rate[w];
