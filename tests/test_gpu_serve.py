@@ -138,3 +138,30 @@ def test_failed_request_reported():
             f, st, _ = sc.solve(r[3])
         assert done[rid][1]["steps"] == st["steps"]
         assert np.all(rownorm_rel(done[rid][0], f) < 1e-10)
+
+
+def test_cancel_while_building():
+    """Cancelling requests right after submission (their material tables may
+    still be building on the server's threads) leaves the others to finish
+    with their results, and the cancelled ids unknown."""
+    fac = serve.RequestFactory(nmin=1e4, nmax=3e4, families=("slab", "bar"))
+    reqs = [fac.make(r) for r in range(1, 25)]
+    with engine.Server(step_bundles=60_000) as sv:
+        ids = [sv.submit(*r[1:]) for r in reqs]
+        gone = ids[5::6]
+        for rid in gone:
+            sv.cancel(rid)
+        keep = [rid for rid in ids if rid not in gone]
+        res = drain(sv, keep)
+        assert sorted(res) == sorted(keep)
+        for rid in gone:
+            with pytest.raises(engine.EngineError) as e:
+                sv.poll(rid)
+            assert e.value.code == abi.MCG_ENOTFOUND
+    for rid, r in zip(ids, reqs):
+        if rid in gone:
+            continue
+        with engine.Scene(r[1], r[2]) as sc:
+            f, st, _ = sc.solve(r[3])
+        assert res[rid][1]["steps"] == st["steps"]
+        assert np.all(rownorm_rel(res[rid][0], f) < 1e-10)
