@@ -3114,7 +3114,14 @@ void build_worker(mcg_server* sv) {
             r = sv->todo.front();
             sv->todo.pop_front();
         }
-        const int rc = build_request(r);
+        int rc;
+        try {
+            rc = build_request(r);
+        } catch (const std::bad_alloc&) {   // nothing may escape a build thread
+            rc = set_error(MCG_ENOMEM, "out of host memory building a request");
+        } catch (const std::exception& e) {
+            rc = set_error(MCG_EINVAL, std::string("building a request: ") + e.what());
+        }
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             r->built = rc ? -1 : 1;
