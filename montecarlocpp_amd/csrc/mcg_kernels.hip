@@ -137,6 +137,12 @@ struct KParams {
     int64_t trace_id;      // bundle whose steps are recorded, -1 for none
     double* trace;         // MCG_TRACE_WIDTH doubles per step
     int64_t trace_cap;     // records available
+    // a tally in global memory, privatised: workgroup w adds into copy
+    // w % ncopies (copy 0 = direct / diff, copy c > 0 = priv + (c - 1) *
+    // copy_stride: direct, then diff); ncopies <= 1: one copy
+    double* priv;
+    int64_t copy_stride;
+    int32_t ncopies;
 };
 
 // Tally sink: LDS when staged, global otherwise (both FP64 atomic adds).
@@ -727,8 +733,11 @@ __device__ View<F> stage(const KParams<F>& K, unsigned char* lds, int tid, int n
         v.direct = s_direct;
         v.diff = s_diff;
     } else {
-        v.direct = K.direct;
-        v.diff = K.diff;
+        // workgroups are dealt to the XCDs in turn, so copy w % ncopies keeps
+        // a workgroup's atomics in its own XCD's L2
+        const int c = K.ncopies > 1 ? int(blockIdx.x % unsigned(K.ncopies)) : 0;
+        v.direct = c == 0 ? K.direct : K.priv + size_t(c - 1) * size_t(K.copy_stride);
+        v.diff = c == 0 ? K.diff : v.direct + (K.diff - K.direct);
     }
     v.planes = s_planes;
     v.sdoms = s_sdoms;
@@ -1960,6 +1969,7 @@ struct mcg_scene {
     // counters, read back in one copy into pinned host memory), emitter CDF
     // (uploaded again only when it changes), timing events
     mcg::DevBuf<double> ws_tally;
+    mcg::DevBuf<double> ws_priv;     // private copies of a tally left in global memory (one per XCD but the first)
     mcg::DevBuf<int64_t> ws_cdf;
     std::vector<int64_t> ws_cdf_host;
     double* pinned = nullptr;
@@ -2061,10 +2071,25 @@ __global__ void zero_kernel(double* a, size_t na, double* b, size_t nb, unsigned
     }
 }
 
+// Adds the private copies of a global tally into the first: a[i] +=
+// sum over c of priv[c * stride + i], i < n.
+__global__ void fold_copies_kernel(double* a, size_t n, const double* priv, size_t stride, int ncopies) {
+    const size_t step = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+        double s = 0.0;
+        for (int c = 0; c < ncopies; ++c) s += priv[size_t(c) * stride + i];
+        a[i] += s;
+    }
+}
+
+// Private copies of a tally that stays in global memory (one per XCD)
+constexpr int kTallyCopies = 8;
+
 // Launch shape and LDS plan of a single solve, then the launch on `stream`
 // (K holds the solve's buffers; its LDS and tail fields are set here).
 template <typename F>
-int launch_solve(mcg_scene* sc, const HostProblem& p, KParams<F>& K, hipStream_t stream) {
+int launch_solve(mcg_scene* sc, const HostProblem& p, KParams<F>& K, hipStream_t stream,
+                 bool may_allocate = false) {
     const HostScene& h = sc->host;
     const int rows = int(p.rows);
     const size_t geo = lds_bytes<F>(h, rows, false, false);
@@ -2085,9 +2110,34 @@ int launch_solve(mcg_scene* sc, const HostProblem& p, KParams<F>& K, hipStream_t
     const int64_t needed = (p.count + block - 1) / block;
     const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(sc->num_cu) * blocks_per_cu, needed)));
     K.tail = int64_t(MCG_TAIL_LANES) * grid * block;
+    // a tally left in global memory gets a private copy per XCD (a solve
+    // that may allocate, whose direct and diff arrays are contiguous),
+    // summed into the first after the launch
+    const size_t nd = size_t(p.rows) * size_t(h.cols), nf = size_t(p.rows) * size_t(h.cols + 1);
+    const int ncopies = std::min(kTallyCopies, grid);
+    K.ncopies = 1;
+    K.priv = nullptr;
+    K.copy_stride = 0;
+    if (!tally && may_allocate && ncopies > 1 && K.diff == K.direct + nd) {
+        int rc;
+        if ((rc = sc->ws_priv.reserve(size_t(ncopies - 1) * (nd + nf)))) return rc;
+        const size_t nz = size_t(ncopies - 1) * (nd + nf);
+        hipLaunchKernelGGL(zero_kernel, dim3(int(std::min<size_t>(1024, (nz + 255) / 256 + 1))), dim3(256), 0, stream,
+                           sc->ws_priv.p, nz, static_cast<double*>(nullptr), size_t(0),
+                           static_cast<unsigned long long*>(nullptr), size_t(0));
+        HIP_TRY(hipGetLastError());
+        K.ncopies = ncopies;
+        K.priv = sc->ws_priv.p;
+        K.copy_stride = int64_t(nd + nf);
+    }
     void* args[] = {&K};
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, stream));
     HIP_TRY(hipGetLastError());
+    if (K.ncopies > 1) {
+        hipLaunchKernelGGL(fold_copies_kernel, dim3(int(std::min<size_t>(1024, (nd + nf + 255) / 256))), dim3(256), 0,
+                           stream, K.direct, nd + nf, static_cast<const double*>(K.priv), nd + nf, K.ncopies - 1);
+        HIP_TRY(hipGetLastError());
+    }
     return MCG_OK;
 }
 
@@ -2152,7 +2202,7 @@ int run_solve(mcg_scene* sc, const HostProblem& p, double* out, mcg_counts* coun
     if (!sc->ev0) HIP_TRY(hipEventCreate(&sc->ev0));
     if (!sc->ev1) HIP_TRY(hipEventCreate(&sc->ev1));
     HIP_TRY(hipEventRecord(sc->ev0, 0));
-    if ((rc = launch_solve<F>(sc, p, K, 0))) return rc;
+    if ((rc = launch_solve<F>(sc, p, K, 0, true))) return rc;
     HIP_TRY(hipEventRecord(sc->ev1, 0));
     HIP_TRY(hipMemcpyAsync(sc->pinned, direct, nall * sizeof(double), hipMemcpyDeviceToHost, 0));
     HIP_TRY(hipStreamSynchronize(0));
