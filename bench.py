@@ -20,12 +20,17 @@ deterministic FP64, beside the whole solve on the CPU checker), "porous_film"
 with its own CPU baseline sample and FP32-vs-FP64 errors), "octet" (the
 reference's 42-cell octet-truss unit, FP32 and FP64, with a CPU sample),
 "batched" (256
-independent slab solves in one persistent launch against the same solves run
-one mcg_solve at a time) and "serving" (an open-loop Poisson request stream
-through the continuous-batching server, with p50/p99 latency).
+independent slab solves in one persistent launch, its graph replay and the
+same solves served by the continuous-batching server, against the same
+solves run one mcg_solve at a time), "serving" (open-loop Poisson request
+streams through the server at six offered loads, with p50/p99 latency, the
+same mix as one batch and one solve at a time, a saturating stream of small
+requests, and FP64 re-solves of 1 % of the requests on the GPU and the CPU
+checker) and "engines" (the history kernel against the event engine on the
+bar, the porous film and the octet truss).
 --headline-only skips the secondary objects and the CPU leg.
 
-    python bench.py --only slab|porous_film|octet|batched|serving
+    python bench.py --only slab|porous_film|octet|batched|serving|engines
 
 runs one secondary object alone (for profiling it).
 
