@@ -17,7 +17,8 @@ sample of the same workload on the host cores, and secondary objects:
 "slab" (the 1-D slab of 1e5 bundles that the reference runs on its CPU, in
 deterministic FP64, beside the whole solve on the CPU checker), "porous_film"
 (the periodic porous film with a cylindrical pore, 1e8 bundles, 64 x 64 map,
-with its own CPU baseline sample and FP32-vs-FP64 errors), "octet" (the
+with its own CPU baseline sample, FP32-vs-FP64 errors and the FP64 rate),
+"octet" (the
 reference's 42-cell octet-truss unit, FP32 and FP64, with a CPU sample),
 "batched" (256
 independent slab solves in one persistent launch, its graph replay and the
@@ -286,7 +287,7 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=1_000_000):
         # warm-up, kept for the precision split: the same 1e7-bundle streams
         # in FP64 (the deterministic mode, equal to the FP64 oracle)
         f_warm32, _, _ = sc.solve(workloads.porous_film(nbundles // 10)[2])
-        f_warm64, _, _ = sc.solve(workloads.porous_film(nbundles // 10, precision=abi.FP64)[2])
+        f_warm64, st64, _ = sc.solve(workloads.porous_film(nbundles // 10, precision=abi.FP64)[2])
         # the statistical noise of a 1e7-bundle solve, for scale: FP32, another seed
         f_seed2, _, _ = sc.solve(workloads.porous_film(nbundles // 10, seed=2)[2])
         runs = []
@@ -320,7 +321,11 @@ def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=1_000_000):
                                            - 1.0),
                "flux_x_map_seed1_vs_seed2": rel(f_warm32[1], f_seed2[1]),
                "temperature_map_seed1_vs_seed2": rel(f_warm32[0], f_seed2[0])},
-           "roofline_frac_96B": steps / kern * BYTES_PER_STEP_FP32 / HBM_PEAK}
+           "roofline_frac_96B": steps / kern * BYTES_PER_STEP_FP32 / HBM_PEAK,
+           # the deterministic mode's rate on the same film: the 1e7-bundle
+           # FP64 solve above (one launch, its HIP-event time)
+           "fp64_1e7_bundles": {"kernel_ms": st64["kernel_ms"], "scatter_steps": st64["steps"],
+                                "kernel_steps_per_s": st64["steps"] / (st64["kernel_ms"] * 1e-3)}}
     if cpu:
         cp = workloads.porous_film(cpu_sample, precision=abi.FP64)[2]
         rate, csteps, secs = cpu_median([(mat, dom, cp)])

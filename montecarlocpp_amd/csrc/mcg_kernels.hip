@@ -54,15 +54,18 @@ constexpr size_t kLdsTallyCap = 48 * 1024;
 constexpr uint32_t kPosRedrawBlock = 64;
 constexpr uint32_t kMaxPosRedraw = 4096;
 
-// FP32 flight on scenes with 2-D / 3-D tallies defers the tally walks
-// (kernels with kDefer): a wavefront's lanes cross very different numbers of
+// Scenes with 2-D / 3-D tallies defer the tally walks (kernels with
+// kDefer): a wavefront's lanes cross very different numbers of
 // cells per step (14 on average, 56 for the longest of 64 lanes on the
 // 64 x 64 porous-film map), so walking them to completion inside the step
 // leaves most lanes idle.  Deferred, a lane walks in the wavefront's walk
 // phase, which runs while at least kWalkQuorum lanes still walk; lanes whose
-// walks are done take their next step meanwhile.  FP64 (the deterministic
-// mode) and scenes without grids walk at once: the same additions, without
-// the walk state's registers held across steps.
+// walks are done take their next step meanwhile.  FP32 flight defers every
+// such walk; FP64 (the deterministic mode) only two-axis walks of single
+// solves, whose state fits its registers (the octet truss's short three-axis
+// walks ran slower deferred, and the FP64 batch kernel would spill).  Other
+// walks run at once: the same additions, without the walk state's registers
+// held across steps.
 // Lanes whose bundle ended claim and emit new bundles together: emission is
 // a long stretch of code (three Philox blocks, the emitter search, the flux
 // alias draw, direction and free path), and run whenever any one lane of a
@@ -399,40 +402,41 @@ __device__ __forceinline__ bool walk_next(Walk<F>& W, const Sink& sink) {
     return false;
 }
 
-// The deferred 2-D / 3-D walk of FP32 flight (NA = 2 grid axes when every
-// grid of the scene is 2-D, else 3): the crossings taken one per call, the
-// axis whose next crossing comes first advancing (an axis with none left
-// holds parameter 2).  Crossings of two axes at the same parameter are taken
-// one after the other, the first with a zero-length share; the statistical
-// comparison of FP32 flight does not see the difference from the merged map
-// entry of field.cpp:156-218, which FP64 keeps.  The amounts are FP64 (exact:
-// a sign times an FP32 value), so a share costs one FP64 multiply; the row
-// stride and row count ride in the walk's registers, so the loop reads no
-// kernel parameter.
-template <int NA>
+// The deferred 2-D / 3-D walk (NA = 2 grid axes when every grid of the scene
+// is 2-D, else 3): the crossings taken one per call, the axis whose next
+// crossing comes first advancing (an axis with none left holds parameter 2).
+// An axis's crossings after its first are reached by increments of |1/dc|.
+// Crossings of two axes at the same parameter are taken one after the other,
+// the first with a zero-length share, which adds nothing: the cells and
+// shares are those of the merged map entries of field.cpp:156-218 up to the
+// rounding of the increments (FP64: well inside the 1e-5 tally comparison).
+// The amounts are FP64 (exact in FP32 flight: a sign times an FP32 value),
+// so a share costs one FP64 multiply; the row stride and row count ride in
+// the walk's registers, so the loop reads no kernel parameter.
+template <typename F, int NA>
 struct WalkN {
-    float t[NA];             // next crossing parameter per axis, 2 once none is left
-    float dt[NA];            // parameter step between the axis's crossings
+    F t[NA];                 // next crossing parameter per axis, 2 once none is left
+    F dt[NA];                // parameter step between the axis's crossings
     int32_t n[NA];           // crossings left per axis
     int32_t dcol[NA];        // column step of a crossing
     int32_t idx;             // flat index of the current cell's first row
     int32_t rstride;         // row stride (the tally's columns)
     int32_t nv;              // amounts (rows) of the step
-    float prev;              // parameter reached
+    F prev;                  // parameter reached
     double amt[kMaxRows];
 };
 
 // One map entry of the deferred walk; true after the last segment.
-template <int NA>
-__device__ __forceinline__ bool walk_next(WalkN<NA>& W, const Sink& sink) {
+template <typename F, int NA>
+__device__ __forceinline__ bool walk_next(WalkN<F, NA>& W, const Sink& sink) {
     // the parameters are ordinary numbers: a compare and select, without the
-    // NaN canonicalisation fminf costs
-    float tm = W.t[0];
+    // NaN canonicalisation fmin costs
+    F tm = W.t[0];
 #pragma unroll
     for (int q = 1; q < NA; ++q) tm = W.t[q] < tm ? W.t[q] : tm;
-    const bool done = tm == 2.0f;
-    if (!done || W.prev != 1.0f) {
-        const double seg = double((done ? 1.0f : tm) - W.prev);
+    const bool done = tm == F(2);
+    if (!done || W.prev != F(1)) {
+        const double seg = double((done ? F(1) : tm) - W.prev);
         double* p = sink.direct + W.idx;
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j) {
@@ -452,14 +456,14 @@ __device__ __forceinline__ bool walk_next(WalkN<NA>& W, const Sink& sink) {
         const int32_t n = W.n[q] - 1;
         W.idx += take ? W.dcol[q] : 0;
         W.n[q] = take ? n : W.n[q];
-        W.t[q] = take ? (n > 0 ? W.t[q] + W.dt[q] : 2.0f) : W.t[q];
+        W.t[q] = take ? (n > 0 ? W.t[q] + W.dt[q] : F(2)) : W.t[q];
     }
     return false;
 }
 
 template <typename F, int kDefer>
 struct WalkSel {
-    using type = WalkN<kDefer>;
+    using type = WalkN<F, kDefer>;
 };
 template <typename F>
 struct WalkSel<F, 0> {
@@ -469,10 +473,9 @@ template <typename F, int kDefer>
 using WalkOf = typename WalkSel<F, kDefer>::type;
 
 // Field::accumulate (field.cpp:92-220) of the amounts `famt` (nv entries on
-// rows r0..) over the segment a -> b of subdomain S.  With kDefer (FP32
-// flight) a 2-D / 3-D walk is only set up in W (the caller runs walk_next
-// later) and the function returns true; otherwise everything is added before
-// it returns.
+// rows r0..) over the segment a -> b of subdomain S.  With kDefer a 2-D /
+// 3-D walk is only set up in W (the caller runs walk_next later) and the
+// function returns true; otherwise everything is added before it returns.
 template <typename F, int kDefer>
 __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink,
                                            const dev::V3<F>& a, const dev::V3<F>& b, int r0,
@@ -550,7 +553,6 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
         ei[k] = min(max(ve, 0), S.max[k]);
     }
     if constexpr (kDefer != 0) {
-        static_assert(sizeof(F) == 4, "deferred walks are an FP32 flight path");
         // the crossings of each walked axis: first parameter, step, count
         int col = S.col_off;
 #pragma unroll
@@ -564,18 +566,30 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             const int ek = k == 0 ? ei[0] : (k == 1 ? ei[1] : ei[2]);
             const int sk = k == 0 ? S.stride[0] : (k == 1 ? S.stride[1] : S.stride[2]);
             const F dc = e0 - b0;
-            const bool none = fabsf(dc) < FLT_MIN || bk == ek;
-            const F r = __builtin_amdgcn_rcpf(dc);
             const bool up = bk < ek;
-            W.t[q] = none ? 2.0f : (F(up ? bk + 1 : bk) - b0) * r;
-            W.dt[q] = fabsf(r);
-            W.n[q] = none ? 0 : (up ? ek - bk : bk - ek);
+            if constexpr (sizeof(F) == 4) {
+                const bool none = fabsf(dc) < FLT_MIN || bk == ek;
+                const F r = __builtin_amdgcn_rcpf(dc);
+                W.t[q] = none ? F(2) : (F(up ? bk + 1 : bk) - b0) * r;
+                W.dt[q] = fabsf(r);
+                W.n[q] = none ? 0 : (up ? ek - bk : bk - ek);
+            } else {
+                // FP64: the first crossing as the checker computes it, later
+                // ones by increments of |1/dc| (the shares move by rounding
+                // only; a tie of two axes becomes two entries, the first of
+                // zero length)
+                const bool none = fabs(dc) < DBL_MIN || bk == ek;
+                const F dcs = none ? F(1) : dc;
+                W.t[q] = none ? F(2) : (F(up ? bk + 1 : bk) - b0) / dcs;
+                W.dt[q] = F(1) / fabs(dcs);
+                W.n[q] = none ? 0 : (up ? ek - bk : bk - ek);
+            }
             W.dcol[q] = up ? sk : -sk;
         }
         W.idx = r0 * sink.cols + col;
         W.rstride = sink.cols;
         W.nv = nv;
-        W.prev = 0.0f;
+        W.prev = F(0);
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j) W.amt[j] = double(famt[j]);
         return true;
@@ -1218,14 +1232,14 @@ __device__ __forceinline__ bool emit_now(bool need, bool busy) {
 // end, so the wavefront's loop always progresses (lanes waiting to emit do
 // not count: emission waits for the walkers).
 // U crossings of the walking lanes (a walk that ends stops there)
-template <int U, int NA>
-__device__ __forceinline__ void walk_some(WalkN<NA>& W, bool& walking, const Sink& sink) {
+template <int U, typename F, int NA>
+__device__ __forceinline__ void walk_some(WalkN<F, NA>& W, bool& walking, const Sink& sink) {
     if (walking && walk_next(W, sink)) walking = false;
     if constexpr (U > 1) walk_some<U - 1>(W, walking, sink);
 }
 
-template <int NA>
-__device__ __forceinline__ void walk_phase(WalkN<NA>& W, bool& walking, bool has_work, const Sink& sink) {
+template <typename F, int NA>
+__device__ __forceinline__ void walk_phase(WalkN<F, NA>& W, bool& walking, bool has_work, const Sink& sink) {
     for (;;) {
         const uint64_t m = __ballot(walking);
         if (m == 0ull) break;
@@ -1796,9 +1810,13 @@ enum class KernelKind { Trajectory, Batch };
 
 template <typename F, bool kTables, bool kTally, int kDefer>
 const void* kernel_of(KernelKind kind) {
-    return kind == KernelKind::Trajectory
-               ? reinterpret_cast<const void*>(trajectory_kernel<F, kTables, kTally, kDefer>)
-               : reinterpret_cast<const void*>(batch_kernel<F, kTables, kTally, kDefer>);
+    // FP64 defers walks in single solves only: no such batch kernel is built
+    if constexpr (sizeof(F) == 8 && kDefer != 0)
+        return reinterpret_cast<const void*>(trajectory_kernel<F, kTables, kTally, kDefer>);
+    else
+        return kind == KernelKind::Trajectory
+                   ? reinterpret_cast<const void*>(trajectory_kernel<F, kTables, kTally, kDefer>)
+                   : reinterpret_cast<const void*>(batch_kernel<F, kTables, kTally, kDefer>);
 }
 
 template <typename F, int kDefer>
@@ -1807,12 +1825,15 @@ const void* kernel_by_plan(KernelKind kind, bool tables, bool tally) {
                   : (tally ? kernel_of<F, false, true, kDefer>(kind) : kernel_of<F, false, false, kDefer>(kind));
 }
 
-// The kernel variant for an LDS plan; deferred walks only for FP32 flight on
-// scenes with 2-D / 3-D tallies, over two axes when no grid is 3-D.
+// The kernel variant for an LDS plan.  Deferred walks on scenes with 2-D /
+// 3-D tallies: in FP32 flight over two axes when no grid is 3-D, else three;
+// in FP64 only for single solves whose grids are all 2-D.
 template <typename F>
 const void* pick_kernel(KernelKind kind, bool tables, bool tally, bool grid, bool grid_2d) {
     if constexpr (sizeof(F) == 4) {
         if (grid) return grid_2d ? kernel_by_plan<F, 2>(kind, tables, tally) : kernel_by_plan<F, 3>(kind, tables, tally);
+    } else {
+        if (grid && grid_2d && kind == KernelKind::Trajectory) return kernel_by_plan<F, 2>(kind, tables, tally);
     }
     return kernel_by_plan<F, 0>(kind, tables, tally);
 }
