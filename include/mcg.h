@@ -326,7 +326,13 @@ struct mcg_tune {
     int32_t engine;            /* mcg_engine (single solves, mcg_solve)         */
     int32_t compact_every;     /* event engine: rounds between regroupings, the
                                   others keep bundles in place (0 = 1)         */
-    int32_t reserved;
+    int32_t defer_walks;       /* history engine, scenes with 2-D / 3-D tally
+                                  grids: a step's crossing walk deferred to the
+                                  wavefront's walk phase, 1 yes, -1 no (walked
+                                  inside the step), 0 auto (yes when some grid
+                                  has at least 32 cells).  FP64 defers only in
+                                  single solves whose grids are all 2-D.
+                                  Results differ by rounding only.            */
     int64_t live;              /* event engine: bundles in flight (0 = auto)    */
 };
 

@@ -109,7 +109,7 @@ ENGINE_HISTORY, ENGINE_EVENT = range(2)
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("per_thread", C.c_int32), ("chunk", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("lds_tables", C.c_int32), ("engine", C.c_int32),
-                ("compact_every", C.c_int32), ("reserved", C.c_int32), ("live", C.c_int64)]
+                ("compact_every", C.c_int32), ("defer_walks", C.c_int32), ("live", C.c_int64)]
 
 
 def dptr(a):

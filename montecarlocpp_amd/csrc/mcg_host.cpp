@@ -437,6 +437,7 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
     s.pores.clear();
     s.has_1d = s.has_grid = false;
     s.grid_2d = true;
+    s.grid_cells = 0;
     const int ns = spec->nsdom;
     std::vector<double> face_temp;            // pre-pairing temperature per plane
     std::vector<double> face_area;
@@ -508,6 +509,9 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             d.accum = positive + 1;
             s.has_grid = true;
             s.grid_2d = s.grid_2d && positive == 2;
+            double cells = 1.0;
+            for (int k = 0; k < 3; ++k) cells *= double(std::max<int64_t>(div[k], 1));
+            s.grid_cells = std::max<int64_t>(s.grid_cells, int64_t(std::min(cells, 1e18)));
             int g = 0;
             for (int k = 0; k < 3; ++k)
                 if (div[k] > 0 && g < 2) d.gax[g++] = k;

@@ -1825,15 +1825,33 @@ const void* kernel_by_plan(KernelKind kind, bool tables, bool tally) {
                   : (tally ? kernel_of<F, false, true, kDefer>(kind) : kernel_of<F, false, false, kDefer>(kind));
 }
 
-// The kernel variant for an LDS plan.  Deferred walks on scenes with 2-D /
-// 3-D tallies: in FP32 flight over two axes when no grid is 3-D, else three;
-// in FP64 only for single solves whose grids are all 2-D.
+// Grids smaller than this walk inside the step unless deferral is asked for:
+// a walk over a few cells (the octet truss's struts, 4-8 cells) ends in a
+// crossing or two, and the walk phase's set-up and checks cost more than the
+// idle lanes they save (FP32 octet 35.4 ms in-step, 37.8 ms deferred; a
+// 7 x 5 x 3 map of the porous film 36.5 ms in-step, 34.8 ms deferred).
+constexpr int64_t kDeferMinCells = 32;
+
+// Axes of a launch's deferred tally walks (0: each walk runs inside its
+// step).  Only scenes with 2-D / 3-D grids defer, over two axes when no grid
+// is 3-D; FP64 only in single solves whose grids are all 2-D (no other FP64
+// deferred kernel is built).  `knob` is mcg_tune.defer_walks: -1 never, 1
+// wherever such a kernel exists, 0 when some grid has kDeferMinCells cells.
 template <typename F>
-const void* pick_kernel(KernelKind kind, bool tables, bool tally, bool grid, bool grid_2d) {
+int walk_axes(bool grid, bool grid_2d, int64_t cells, KernelKind kind, int knob) {
+    if (!grid || knob < 0 || (knob == 0 && cells < kDeferMinCells)) return 0;
+    if constexpr (sizeof(F) == 8) return grid_2d && kind == KernelKind::Trajectory ? 2 : 0;
+    return grid_2d ? 2 : 3;
+}
+
+// The kernel variant for an LDS plan and walk_axes' choice.
+template <typename F>
+const void* pick_kernel(KernelKind kind, bool tables, bool tally, int walk) {
     if constexpr (sizeof(F) == 4) {
-        if (grid) return grid_2d ? kernel_by_plan<F, 2>(kind, tables, tally) : kernel_by_plan<F, 3>(kind, tables, tally);
+        if (walk == 2) return kernel_by_plan<F, 2>(kind, tables, tally);
+        if (walk == 3) return kernel_by_plan<F, 3>(kind, tables, tally);
     } else {
-        if (grid && grid_2d && kind == KernelKind::Trajectory) return kernel_by_plan<F, 2>(kind, tables, tally);
+        if (walk == 2 && kind == KernelKind::Trajectory) return kernel_by_plan<F, 2>(kind, tables, tally);
     }
     return kernel_by_plan<F, 0>(kind, tables, tally);
 }
@@ -2121,7 +2139,9 @@ int launch_solve(mcg_scene* sc, const HostProblem& p, KParams<F>& K, hipStream_t
     const size_t lds = lp.bytes;
     if (lds > sc->lds_limit) return set_error(MCG_ERANGE, "geometry exceeds the LDS budget");
     K.lds_tally = tally ? 1 : 0;
-    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally, h.has_grid, h.grid_2d);
+    const void* fn = pick_kernel<F>(KernelKind::Trajectory, tables, tally,
+                                    walk_axes<F>(h.has_grid, h.grid_2d, h.grid_cells, KernelKind::Trajectory,
+                                                 sc->tune.defer_walks));
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     int blocks_per_cu = std::max<int>(1, int(sc->lds_limit / std::max<size_t>(lds, 1)));
     // one workgroup per CU (a large LDS tally): give it the CU's 16 waves
@@ -2648,8 +2668,7 @@ struct BatchPlan {
     size_t n_direct = 0, n_diff = 0;
     size_t lds = 0;
     bool tables = true, tally = true;
-    bool grid_tally = false;   // some solve tallies on a 2-D / 3-D grid
-    bool grid_2d = true;       // no solve's grid is 3-D
+    int walk = 0;              // axes of deferred tally walks (walk_axes)
     int block = 256, grid = 0, chunk = kChunk;
 };
 
@@ -2659,7 +2678,8 @@ int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
                BatchPlan& B) {
     const mcg_scene* s0 = scenes[0];
     size_t geo = 0, tab = 0, tally_bytes = 0;
-    bool grid = false;
+    bool grid = false, grid_2d = true;
+    int64_t cells = 0;
     for (int i = 0; i < n; ++i) {
         const HostScene& h = scenes[i]->host;
         if (scenes[i]->device != s0->device) return set_error(MCG_EINVAL, "batch scenes on different devices");
@@ -2668,10 +2688,13 @@ int plan_shape(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
         tab = std::max(tab, lds_bytes<F>(h, int(P[i].rows), true, false) - g);
         tally_bytes = std::max(tally_bytes, lds_bytes<F>(h, int(P[i].rows), false, true) - g);
         grid = grid || h.has_grid;
-        B.grid_2d = B.grid_2d && h.grid_2d;
+        grid_2d = grid_2d && h.grid_2d;
+        cells = std::max(cells, h.grid_cells);
     }
+    if (tune && (tune->defer_walks < -1 || tune->defer_walks > 1))
+        return set_error(MCG_EINVAL, "defer_walks must be -1, 0 or 1");
     const LdsPlan lp = plan_lds(geo, tab, tally_bytes, grid, !(tune && tune->lds_tables < 0), s0->lds_limit);
-    B.grid_tally = grid;
+    B.walk = walk_axes<F>(grid, grid_2d, cells, KernelKind::Batch, tune ? tune->defer_walks : 0);
     B.tables = lp.tables;
     B.tally = lp.tally;
     B.lds = lp.bytes;
@@ -2714,7 +2737,7 @@ int plan_batch(int n, mcg_scene* const* scenes, const HostProblem* P, const mcg_
 // Enqueue one batch launch on `stream`.
 template <typename F>
 int launch_batch(const BatchPlan& B, const KParams<F>* kp, int nsolves, hipStream_t stream) {
-    const void* fn = pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally, B.grid_2d);
+    const void* fn = pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.walk);
     HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
     void* args[] = {&kp, &nsolves};
     HIP_TRY(hipLaunchKernel(fn, dim3(std::max(1, B.grid)), dim3(B.block), args, B.lds, stream));
@@ -3002,7 +3025,7 @@ struct GraphBatch : GraphBatchBase {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         // the kernel's LDS limit is an attribute of the function, set before capture
-        HIP_TRY(hipFuncSetAttribute(pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.grid_tally, B.grid_2d),
+        HIP_TRY(hipFuncSetAttribute(pick_kernel<F>(KernelKind::Batch, B.tables, B.tally, B.walk),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(B.lds)));
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         const size_t total = direct.n + diff.n + ctr.n;
@@ -3530,6 +3553,7 @@ int mcg_scene_tune(mcg_scene* scene, const mcg_tune* tune) {
     if (t.engine != MCG_ENGINE_HISTORY && t.engine != MCG_ENGINE_EVENT)
         return mcg::set_error(MCG_EINVAL, "unknown engine");
     if (t.chunk != 0 && t.chunk < 64) return mcg::set_error(MCG_EINVAL, "chunk must be at least 64");
+    if (t.defer_walks < -1 || t.defer_walks > 1) return mcg::set_error(MCG_EINVAL, "defer_walks must be -1, 0 or 1");
     scene->tune = t;
     return MCG_OK;
 }
@@ -3788,21 +3812,19 @@ int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_
                 x.compact_every = ce;
                 cand.push_back(x);
             }
-    for (const mcg_tune& x : cand) {
+    auto measure = [&](const mcg_tune& x) {
         scene->tune = x;
         mcg_stats st{};
         double ms = 1e300;
         for (int rep = 0; rep < 2; ++rep) {   // first run warms up the shape
+            int r;
             if (x.engine == MCG_ENGINE_EVENT)
-                rc = p.precision == MCG_FP64 ? mcg::run_solve_event<double>(scene, p, out.data(), nullptr, &st)
-                                             : mcg::run_solve_event<float>(scene, p, out.data(), nullptr, &st);
+                r = p.precision == MCG_FP64 ? mcg::run_solve_event<double>(scene, p, out.data(), nullptr, &st)
+                                            : mcg::run_solve_event<float>(scene, p, out.data(), nullptr, &st);
             else
-                rc = p.precision == MCG_FP64 ? mcg::run_solve<double>(scene, p, out.data(), nullptr, &st)
-                                             : mcg::run_solve<float>(scene, p, out.data(), nullptr, &st);
-            if (rc) {
-                scene->tune = saved;
-                return rc;
-            }
+                r = p.precision == MCG_FP64 ? mcg::run_solve<double>(scene, p, out.data(), nullptr, &st)
+                                            : mcg::run_solve<float>(scene, p, out.data(), nullptr, &st);
+            if (r) return r;
             ms = std::min(ms, st.kernel_ms);
         }
         const double rate = double(st.steps) / (ms * 1e-3);
@@ -3810,6 +3832,23 @@ int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_
             top_rate = rate;
             top = x;
         }
+        return int(MCG_OK);
+    };
+    for (const mcg_tune& x : cand)
+        if ((rc = measure(x))) break;
+    // a scene with 2-D / 3-D grids: the best history shape with its tally
+    // walks deferred and walked inside the step
+    if (!rc && scene->host.has_grid && top.engine == MCG_ENGINE_HISTORY) {
+        const mcg_tune shape = top;
+        for (int d : {1, -1}) {
+            mcg_tune x = shape;
+            x.defer_walks = d;
+            if ((rc = measure(x))) break;
+        }
+    }
+    if (rc) {
+        scene->tune = saved;
+        return rc;
     }
     scene->tune = top;
     if (best) *best = top;
@@ -3839,9 +3878,29 @@ int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_sp
     std::vector<mcg_stats> st(n);
     mcg_tune top{};
     double top_rate = -1.0;
+    auto measure = [&](const mcg_tune& cand) {
+        double ms = 1e300;
+        int64_t steps = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            int rc = prec == MCG_FP64
+                         ? mcg::run_batch<double>(n, sc.data(), P.data(), optr.data(), st.data(), &cand)
+                         : mcg::run_batch<float>(n, sc.data(), P.data(), optr.data(), st.data(), &cand);
+            if (rc) return rc;
+            ms = std::min(ms, st[0].kernel_ms);
+            steps = 0;
+            for (const mcg_stats& x : st) steps += x.steps;
+        }
+        const double rate = double(steps) / (ms * 1e-3);
+        if (rate > top_rate) {
+            top_rate = rate;
+            top = cand;
+        }
+        return int(MCG_OK);
+    };
     static const int blocks[] = {128, 256, 512, 1024};
     static const int tables[] = {1, -1};
     static const int chunks[] = {64, 256, 1024};
+    int rc;
     for (int b : blocks)
         for (int t : tables)
             for (int c : chunks) {
@@ -3849,23 +3908,20 @@ int mcg_autotune_batch(int32_t n, mcg_scene* const* scenes, const mcg_problem_sp
                 cand.block = b;
                 cand.chunk = c;
                 cand.lds_tables = t;
-                double ms = 1e300;
-                int64_t steps = 0;
-                for (int rep = 0; rep < 2; ++rep) {
-                    int rc = prec == MCG_FP64
-                                 ? mcg::run_batch<double>(n, sc.data(), P.data(), optr.data(), st.data(), &cand)
-                                 : mcg::run_batch<float>(n, sc.data(), P.data(), optr.data(), st.data(), &cand);
-                    if (rc) return rc;
-                    ms = std::min(ms, st[0].kernel_ms);
-                    steps = 0;
-                    for (const mcg_stats& x : st) steps += x.steps;
-                }
-                const double rate = double(steps) / (ms * 1e-3);
-                if (rate > top_rate) {
-                    top_rate = rate;
-                    top = cand;
-                }
+                if ((rc = measure(cand))) return rc;
             }
+    // FP32 batches with 2-D / 3-D grids: the best shape with its tally walks
+    // deferred and walked inside the step (FP64 batches walk inside the step)
+    bool grid = false;
+    for (int i = 0; i < n; ++i) grid = grid || scenes[i]->host.has_grid;
+    if (grid && prec != MCG_FP64) {
+        const mcg_tune shape = top;
+        for (int d : {1, -1}) {
+            mcg_tune cand = shape;
+            cand.defer_walks = d;
+            if ((rc = measure(cand))) return rc;
+        }
+    }
     if (best) *best = top;
     if (best_steps_per_s) *best_steps_per_s = top_rate;
     return MCG_OK;

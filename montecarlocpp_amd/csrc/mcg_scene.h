@@ -125,6 +125,7 @@ struct HostScene {
     bool has_1d = false;     // some subdomain tallies along one axis (range adds)
     bool has_grid = false;   // some subdomain tallies on a 2-D / 3-D grid
     bool grid_2d = false;    // no grid is 3-D (with has_grid: every grid is 2-D, accum 3)
+    int64_t grid_cells = 0;  // cells of the largest 2-D / 3-D grid
 };
 
 struct HostProblem {

@@ -89,7 +89,7 @@ def test_event_engine_fp32_equals_history(label):
 def test_event_tune_rejects_bad_values():
     with engine.Scene(materials.si_like(300.0), scenes.slab(1e-7, 10)) as sc:
         for kw, msg in ((dict(per_thread=3), "per_thread"), (dict(engine=7), "unknown engine"),
-                        (dict(compact_every=-1), "negative")):
+                        (dict(compact_every=-1), "negative"), (dict(defer_walks=2), "defer_walks")):
             t = abi.Tune()
             for k, v in kw.items():
                 setattr(t, k, v)
