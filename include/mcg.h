@@ -331,8 +331,8 @@ struct mcg_tune {
                                   wavefront's walk phase, 1 yes, -1 no (walked
                                   inside the step), 0 auto (yes when some grid
                                   has at least 32 cells).  FP64 defers only in
-                                  single solves whose grids are all 2-D.
-                                  Results differ by rounding only.            */
+                                  single solves.  Results differ by rounding
+                                  only.                                       */
     int64_t live;              /* event engine: bundles in flight (0 = auto)    */
 };
 

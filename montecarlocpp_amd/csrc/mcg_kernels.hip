@@ -60,12 +60,10 @@ constexpr uint32_t kMaxPosRedraw = 4096;
 // 64 x 64 porous-film map), so walking them to completion inside the step
 // leaves most lanes idle.  Deferred, a lane walks in the wavefront's walk
 // phase, which runs while at least kWalkQuorum lanes still walk; lanes whose
-// walks are done take their next step meanwhile.  FP32 flight defers every
-// such walk; FP64 (the deterministic mode) only two-axis walks of single
-// solves, whose state fits its registers (the octet truss's short three-axis
-// walks ran slower deferred, and the FP64 batch kernel would spill).  Other
-// walks run at once: the same additions, without the walk state's registers
-// held across steps.
+// walks are done take their next step meanwhile.  Grids of a few cells walk
+// at once (kDeferMinCells), and so does the FP64 batch kernel, which would
+// spill the walk state: the same additions, without the walk state's
+// registers held across steps.
 // Lanes whose bundle ended claim and emit new bundles together: emission is
 // a long stretch of code (three Philox blocks, the emitter search, the flux
 // alias draw, direction and free path), and run whenever any one lane of a
@@ -1834,13 +1832,13 @@ constexpr int64_t kDeferMinCells = 32;
 
 // Axes of a launch's deferred tally walks (0: each walk runs inside its
 // step).  Only scenes with 2-D / 3-D grids defer, over two axes when no grid
-// is 3-D; FP64 only in single solves whose grids are all 2-D (no other FP64
-// deferred kernel is built).  `knob` is mcg_tune.defer_walks: -1 never, 1
-// wherever such a kernel exists, 0 when some grid has kDeferMinCells cells.
+// is 3-D; FP64 only in single solves (no FP64 batch kernel that defers is
+// built).  `knob` is mcg_tune.defer_walks: -1 never, 1 wherever such a
+// kernel exists, 0 when some grid has kDeferMinCells cells.
 template <typename F>
 int walk_axes(bool grid, bool grid_2d, int64_t cells, KernelKind kind, int knob) {
     if (!grid || knob < 0 || (knob == 0 && cells < kDeferMinCells)) return 0;
-    if constexpr (sizeof(F) == 8) return grid_2d && kind == KernelKind::Trajectory ? 2 : 0;
+    if (sizeof(F) == 8 && kind != KernelKind::Trajectory) return 0;
     return grid_2d ? 2 : 3;
 }
 
@@ -1851,7 +1849,8 @@ const void* pick_kernel(KernelKind kind, bool tables, bool tally, int walk) {
         if (walk == 2) return kernel_by_plan<F, 2>(kind, tables, tally);
         if (walk == 3) return kernel_by_plan<F, 3>(kind, tables, tally);
     } else {
-        if (walk == 2 && kind == KernelKind::Trajectory) return kernel_by_plan<F, 2>(kind, tables, tally);
+        if (kind == KernelKind::Trajectory && walk == 2) return kernel_by_plan<F, 2>(kind, tables, tally);
+        if (kind == KernelKind::Trajectory && walk == 3) return kernel_by_plan<F, 3>(kind, tables, tally);
     }
     return kernel_by_plan<F, 0>(kind, tables, tally);
 }

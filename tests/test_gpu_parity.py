@@ -161,8 +161,8 @@ def test_grid_walk_conserves_amounts(si300, precision, div):
     """A tally grid only splits each step's amounts over the cells it
     crosses: on the same streams, the volume-weighted sum of a grid tally
     equals the single-cell tally of the same scene (FP32 flight walks its
-    2-D / 3-D tallies deferred, in the wavefront's walk phase, and FP64 its
-    2-D ones; FP64 walks the 3-D grid inside the step)."""
+    2-D / 3-D tallies deferred, in the wavefront's walk phase, FP64 too in
+    single solves; grids under 32 cells walk inside the step)."""
     grid = scenes.porous_film(div=div)
     whole = scenes.porous_film(div=(0, 0, 0))
     prob = scenes.problem(100_000, maxscat=60, seed=21, precision=precision)
