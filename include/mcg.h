@@ -370,9 +370,11 @@ int mcg_batch_destroy(mcg_batch* batch);
  * problem, and returns; the server's own host threads build the material
  * tables, the problem and the request's packed device image meanwhile.
  * mcg_server_step waits for a request's build when it reaches the head of
- * the queue and admits it with one device allocation and one copy.  A
- * request whose material or problem is invalid, or that could never fit the
- * budget, is listed by mcg_server_completed and its error is returned by
+ * the queue and admits it with one device allocation and one copy (blocks
+ * of retired requests are freed lazily, when no step is in flight, or at
+ * once when that allocation fails).  A request whose material or problem is
+ * invalid, that could never fit the budget, or whose allocation or upload
+ * failed, is listed by mcg_server_completed and its error is returned by
  * mcg_server_poll. */
 typedef struct mcg_server mcg_server;
 
@@ -406,8 +408,11 @@ int mcg_server_destroy(mcg_server* server);
  * (mcg_scene_tune).  mcg_autotune's candidates are both engines: the history
  * kernel's block (64-1024), table placement (LDS or not) and bundle ids per
  * wavefront grab, and the event engine's bundles in flight, bundles per
- * thread (1, 4) and rounds between regroupings (1, 4).  mcg_autotune_batch
- * tunes the batch kernel's block, table placement and grab.  Only
+ * thread (1, 4) and rounds between regroupings (1, 4); on scenes with 2-D /
+ * 3-D tally grids, then the best history shape with its walks deferred and
+ * in the step (defer_walks 1, -1).  mcg_autotune_batch tunes the batch
+ * kernel's block, table placement and grab (and, for FP32 batches with
+ * grids, defer_walks).  Only
  * scheduling changes, never results.  sample_bundles <= 0 means
  * min(nemit, 2^20); sample_fraction <= 0 means 0.1. */
 int mcg_autotune(mcg_scene* scene, const mcg_problem_spec* prob, int64_t sample_bundles,

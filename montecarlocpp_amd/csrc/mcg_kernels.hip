@@ -3244,12 +3244,31 @@ void stop_workers(mcg_server* sv) {
     sv->workers.clear();
 }
 
+// Free device memory of retired requests (called when nothing is in flight).
+void bury(mcg_server* sv) {
+    for (auto& r : sv->graveyard) {
+        r->scene.reset();
+        if (r->block) (void)hipFree(r->block);
+        r->block = nullptr;
+    }
+    sv->graveyard.clear();
+}
+
 // A request's device side: one block, its image uploaded in one copy.
 int admit(mcg_server* sv, Request* r) {
     const ImageLayout& L = r->lay;
     void* b = nullptr;
-    const hipError_t e = hipMalloc(&b, L.total);
-    if (e != hipSuccess) return set_error(MCG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hipError_t e = hipMalloc(&b, L.total);
+    if (e != hipSuccess && !sv->graveyard.empty()) {
+        // retired requests' blocks are freed lazily: free them now, then retry
+        (void)hipGetLastError();
+        bury(sv);
+        e = hipMalloc(&b, L.total);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(MCG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     r->block = b;
     if (L.upload) HIP_TRY(hipMemcpy(b, r->image.data(), L.upload, hipMemcpyHostToDevice));
     std::vector<unsigned char>().swap(r->image);
@@ -3270,16 +3289,6 @@ int admit(mcg_server* sv, Request* r) {
     sv->used += r->bytes;
     sv->active.push_back(r);
     return MCG_OK;
-}
-
-// Free device memory of retired requests (called when nothing is in flight).
-void bury(mcg_server* sv) {
-    for (auto& r : sv->graveyard) {
-        r->scene.reset();
-        if (r->block) (void)hipFree(r->block);
-        r->block = nullptr;
-    }
-    sv->graveyard.clear();
 }
 
 int retire(mcg_server* sv, Request* r) {
@@ -4027,7 +4036,14 @@ int mcg_server_step(mcg_server* sv, int32_t* active, int32_t* queued, int32_t* c
         }
         if (sv->used + r->bytes > sv->budget) break;
         sv->queue.pop_front();
-        if ((rc = mcg::admit(sv, r))) return rc;
+        if ((rc = mcg::admit(sv, r))) {
+            // the request fails with the error (its block, if any, is freed
+            // once it is polled or cancelled); the others go on
+            r->state = 3;
+            r->err_code = rc;
+            r->err_msg = mcg::t_err;
+            sv->completed.push_back(r->id);
+        }
     }
     // enqueue the next step behind the one in flight, then finish that one
     const int nxt = 1 - sv->cur;
