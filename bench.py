@@ -12,8 +12,9 @@ tallies and sampling tables.  One timed step is one complete field solve
 Printed: one JSON line with the scatter-step throughput, the roofline
 object of the trajectory kernel (algorithmic bytes = 96 B of bundle state
 round trip per scatter-step), the tally error of the FP32-flight run against
-the FP64 run of the same streams, the FP64 CPU checker timed on a bounded
-sample of the same workload on the host cores, and secondary objects:
+the FP64 run of the same streams, the FP64 CPU checker timed on the same
+workload on the host cores (whose result for the first seed the GPU's FP64
+and FP32 runs are also compared with), and secondary objects:
 "slab" (the 1-D slab of 1e5 bundles that the reference runs on its CPU, in
 deterministic FP64, beside the whole solve on the CPU checker), "porous_film"
 (the periodic porous film with a cylindrical pore, 1e8 bundles, 64 x 64 map,
@@ -82,22 +83,27 @@ def cpu_threads():
     return int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
 
 
-def cpu_median(solves, reps=CPU_REPS):
+def cpu_median(solves, reps=CPU_REPS, fields=None):
     """The FP64 checker over (material, domain, problem) solves, reps times:
     the median rate over the reps, each rep's time being the checker's own
     solve timer (steady_clock around the solve, tables and set-up excluded)
-    summed over the solves; the checker runs OpenMP static over bundles."""
+    summed over the solves; the checker runs OpenMP static over bundles.
+    A list passed as `fields` receives the last rep's results."""
     from oracle import oracle  # CPU baseline leg only
     rates, steps, secs = [], 0, []
     for _ in range(reps):
         t = 0.0
         steps = 0
+        out = []
         for mat, dom, prob in solves:
-            _, cst, _ = oracle.solve(mat, dom, prob, threads=cpu_threads())
+            f, cst, _ = oracle.solve(mat, dom, prob, threads=cpu_threads())
             t += cst["kernel_ms"] * 1e-3
             steps += cst["steps"]
+            out.append(f)
         secs.append(t)
         rates.append(steps / t)
+    if fields is not None:
+        fields[:] = out
     return float(np.median(rates)), int(steps), [round(x, 3) for x in secs]
 
 
@@ -204,10 +210,23 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        rate, csteps, secs = cpu_median([(mat, dom, scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64))])
+        fcpu = []
+        rate, csteps, secs = cpu_median([(mat, dom, scenes.problem(NBUNDLES, seed=seed, precision=abi.FP64))],
+                                        fields=fcpu)
         cpu = {"value": rate, "unit": "scatter-steps/s", "cores": cpu_threads(), "kind": "port",
                "sample": f"the whole {NBUNDLES}-bundle bar solve (seed {seed}), FP64, OpenMP static schedule "
                          f"over bundles, median of {CPU_REPS} (solve times {secs} s, {csteps} scatter-steps each)"}
+        # the same solve (seed 1) against the CPU checker's FP64 result: the
+        # GPU's FP64 run (bundle for bundle the same arithmetic) and the FP32
+        # flight run of the same streams
+        fc = fcpu[0]
+        qc = fc[1].mean()
+        tally_error["vs_cpu_oracle_fp64_seed1"] = {
+            "gpu_fp64": {"temperature": rel(f64[0], fc[0]), "heat_flux": abs(f64[1].mean() - qc) / abs(qc),
+                         "k_eff": abs(f64[1].mean() - qc) / abs(qc)},
+            "gpu_fp32_flight": {"temperature": rel(f32[0], fc[0]), "heat_flux": abs(f32[1].mean() - qc) / abs(qc),
+                                "k_eff": abs(f32[1].mean() - qc) / abs(qc)},
+        }
         progress(f"CPU baseline done ({secs} s)")
 
     traffic = None
