@@ -1032,8 +1032,10 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     bool outside;
     if constexpr (sizeof(F) == 4) {
         if (S.box) {
-            outside = L.pos.x < S.kmin[0] || L.pos.x > S.kmax[0] || L.pos.y < S.kmin[1] ||
-                      L.pos.y > S.kmax[1] || L.pos.z < S.kmin[2] || L.pos.z > S.kmax[2];
+            // six compares joined without short-circuits, so that the six
+            // bounds are read together rather than one branch at a time
+            outside = (L.pos.x < S.kmin[0]) | (L.pos.x > S.kmax[0]) | (L.pos.y < S.kmin[1]) |
+                      (L.pos.y > S.kmax[1]) | (L.pos.z < S.kmin[2]) | (L.pos.z > S.kmax[2]);
         } else {
             // a face's distance after the move is d + dmin * rate
             outside = false;
@@ -1048,9 +1050,9 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     } else if (S.box) {
         // the checker's test face by face: n.p + c is p_a - bmin_a or
         // bmax_a - p_a exactly (the normal's other components are zeros)
-        outside = L.pos.x - S.bmin[0] < -S.eps || S.bmax[0] - L.pos.x < -S.eps ||
-                  L.pos.y - S.bmin[1] < -S.eps || S.bmax[1] - L.pos.y < -S.eps ||
-                  L.pos.z - S.bmin[2] < -S.eps || S.bmax[2] - L.pos.z < -S.eps;
+        outside = (L.pos.x - S.bmin[0] < -S.eps) | (S.bmax[0] - L.pos.x < -S.eps) |
+                  (L.pos.y - S.bmin[1] < -S.eps) | (S.bmax[1] - L.pos.y < -S.eps) |
+                  (L.pos.z - S.bmin[2] < -S.eps) | (S.bmax[2] - L.pos.z < -S.eps);
         if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
     } else {
         outside = !inside(V.planes, S, V.pores, L.pos);
