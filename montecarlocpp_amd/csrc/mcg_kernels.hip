@@ -917,40 +917,52 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     if constexpr (sizeof(F) == 4) {
         if (S.box) {
             // an axis-aligned box: on each axis only the face ahead can be
-            // met, at (bound - p_a) / u_a
+            // met, at (bound - p_a) / u_a.  The bounds and face numbers are
+            // read together first and the nearest face is kept with selects,
+            // so that the flight waits for one round of LDS reads rather
+            // than two per axis
             const F p[3] = {L.pos.x, L.pos.y, L.pos.z}, u[3] = {L.dir.x, L.dir.y, L.dir.z};
+            F lo[3], hi[3];
+            int32_t flo[3], fhi[3];
+            const int32_t pbeg = S.pbeg;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = S.bmin[a];
+                hi[a] = S.bmax[a];
+                flo[a] = pbeg + S.fmin[a];
+                fhi[a] = pbeg + S.fmax[a];
+            }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const bool up = u[a] > F(0);
-                const F ta = ((up ? S.bmax[a] : S.bmin[a]) - p[a]) * __builtin_amdgcn_rcpf(u[a]);
+                const F ta = ((up ? hi[a] : lo[a]) - p[a]) * __builtin_amdgcn_rcpf(u[a]);
                 const F t = ta < F(0) ? F(0) : ta;
-                if (u[a] != F(0) && t < dmin) {
-                    dmin = t;
-                    hit = S.pbeg + (up ? S.fmax[a] : S.fmin[a]);
-                }
+                const bool take = u[a] != F(0) && t < dmin;
+                dmin = take ? t : dmin;
+                hit = take ? (up ? fhi[a] : flo[a]) : hit;
             }
         } else {
             // every lane evaluates every face: a reciprocal is cheaper than
-            // the divergence of skipping the faces it moves away from
+            // the divergence of skipping the faces it moves away from.  A
+            // slot past the cell's last face (every cell has at least four)
+            // reads the cell's first face and is then ignored, so that the
+            // face reads issue together rather than behind a branch each.
+            const int32_t pbeg = S.pbeg, pend = S.pend;
 #pragma unroll
             for (int i = 0; i < kMaxFaces; ++i) {
-                dist[i] = F(1);
-                rate[i] = F(0);
-                const int b = S.pbeg + i;
-                if (b < S.pend) {
-                    const DevPlane<F>& P = V.planes[b];
-                    const F nd = dev::dot3(P.n, L.dir);
-                    const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
-                                                                                __builtin_fmaf(P.n[0], L.pos.x, P.c)));
-                    const F tf = -d * __builtin_amdgcn_rcpf(nd);
-                    const F t = tf < F(0) ? F(0) : tf;
-                    if (nd < F(0) && t < dmin) {
-                        dmin = t;
-                        hit = b;
-                    }
-                    dist[i] = d;
-                    rate[i] = nd;
-                }
+                const int b = pbeg + i;
+                const bool real = b < pend;
+                const DevPlane<F>& P = V.planes[real ? b : pbeg];
+                const F nd = dev::dot3(P.n, L.dir);
+                const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
+                                                                            __builtin_fmaf(P.n[0], L.pos.x, P.c)));
+                const F tf = -d * __builtin_amdgcn_rcpf(nd);
+                const F t = tf < F(0) ? F(0) : tf;
+                const bool take = real && nd < F(0) && t < dmin;
+                dmin = take ? t : dmin;
+                hit = take ? b : hit;
+                dist[i] = real ? d : F(1);
+                rate[i] = real ? nd : F(0);
             }
             // faces beyond the first kMaxFaces (prisms and pyramids with
             // large bases) are tested again after the move
