@@ -903,6 +903,10 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     walking = false;
     const DevSdom<F>& S = V.sdoms[L.sdom];
     const F vel = V.vel[L.mode];
+    // the cell's fields the flight tests, read once and together (a lone
+    // lane would otherwise wait for each read behind its own branch)
+    const int32_t s_box = S.box, s_pore = S.pore;
+    const F s_eps = S.eps;
     // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
     F dmin = L.scat_next;
     int hit = -1;
@@ -914,8 +918,9 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     // face's rule (reflection, Inter crossing, absorption) applies instead of
     // a loss.  FP64 keeps the checker's arithmetic.
     F dist[kMaxFaces], rate[kMaxFaces];
+    F klo[3], khi[3];   // FP32 box: the kill test's bounds, read with the faces'
     if constexpr (sizeof(F) == 4) {
-        if (S.box) {
+        if (s_box) {
             // an axis-aligned box: on each axis only the face ahead can be
             // met, at (bound - p_a) / u_a.  The bounds and face numbers are
             // read together first and the nearest face is kept with selects,
@@ -931,6 +936,8 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
                 hi[a] = S.bmax[a];
                 flo[a] = pbeg + S.fmin[a];
                 fhi[a] = pbeg + S.fmax[a];
+                klo[a] = S.kmin[a];
+                khi[a] = S.kmax[a];
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -982,7 +989,9 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     } else {
         (void)dist;
         (void)rate;
-        if (S.box) {
+        (void)klo;
+        (void)khi;
+        if (s_box) {
             // the checker's face loop evaluated for the face ahead on each
             // axis only: the axis's other face has n.u >= 0 (exactly: the
             // normal's other components are zeros) and the loop skips it;
@@ -1014,8 +1023,8 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
             }
         }
     }
-    if (S.pore >= 0) {
-        const DevPore<F>& Q = V.pores[S.pore];
+    if (s_pore >= 0) {
+        const DevPore<F>& Q = V.pores[s_pore];
         const F dx = L.pos.x - Q.cx, dy = L.pos.y - Q.cy;
         const F bq = dx * L.dir.x + dy * L.dir.y;
         if (bq < F(0)) {
@@ -1043,33 +1052,32 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
     // kill test (subdomain.cpp:186-190): outside any face by more than eps
     bool outside;
     if constexpr (sizeof(F) == 4) {
-        if (S.box) {
-            // six compares joined without short-circuits, so that the six
-            // bounds are read together rather than one branch at a time
-            outside = (L.pos.x < S.kmin[0]) | (L.pos.x > S.kmax[0]) | (L.pos.y < S.kmin[1]) |
-                      (L.pos.y > S.kmax[1]) | (L.pos.z < S.kmin[2]) | (L.pos.z > S.kmax[2]);
+        if (s_box) {
+            // six compares joined without short-circuits
+            outside = (L.pos.x < klo[0]) | (L.pos.x > khi[0]) | (L.pos.y < klo[1]) | (L.pos.y > khi[1]) |
+                      (L.pos.z < klo[2]) | (L.pos.z > khi[2]);
         } else {
             // a face's distance after the move is d + dmin * rate
             outside = false;
 #pragma unroll
-            for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -S.eps;
+            for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -s_eps;
             for (int b = S.pbeg + kMaxFaces; b < S.pend; ++b) {
                 const DevPlane<F>& P = V.planes[b];
-                outside |= dev::dot3(P.n, L.pos) + P.c < -S.eps;
+                outside |= dev::dot3(P.n, L.pos) + P.c < -s_eps;
             }
         }
-        if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
-    } else if (S.box) {
+        if (s_pore >= 0) outside |= !outside_pore(V.pores[s_pore], L.pos);
+    } else if (s_box) {
         // the checker's test face by face: n.p + c is p_a - bmin_a or
         // bmax_a - p_a exactly (the normal's other components are zeros)
-        outside = (L.pos.x - S.bmin[0] < -S.eps) | (S.bmax[0] - L.pos.x < -S.eps) |
-                  (L.pos.y - S.bmin[1] < -S.eps) | (S.bmax[1] - L.pos.y < -S.eps) |
-                  (L.pos.z - S.bmin[2] < -S.eps) | (S.bmax[2] - L.pos.z < -S.eps);
-        if (S.pore >= 0) outside |= !outside_pore(V.pores[S.pore], L.pos);
+        outside = (L.pos.x - S.bmin[0] < -s_eps) | (S.bmax[0] - L.pos.x < -s_eps) |
+                  (L.pos.y - S.bmin[1] < -s_eps) | (S.bmax[1] - L.pos.y < -s_eps) |
+                  (L.pos.z - S.bmin[2] < -s_eps) | (S.bmax[2] - L.pos.z < -s_eps);
+        if (s_pore >= 0) outside |= !outside_pore(V.pores[s_pore], L.pos);
     } else {
         outside = !inside(V.planes, S, V.pores, L.pos);
     }
-    if (dmin < -S.eps || outside) {
+    if ((dmin < -s_eps) | outside) {
         fate = MCG_ESCAPED;
     } else {
         // tally (problem.cpp:414-416 with the kinds of problem.cpp:473-637);
