@@ -839,11 +839,11 @@ __device__ __forceinline__ int inter_nearest(const View<F>& V, const DevPlane<F>
 // wavefront whose lanes take different events runs the direction code once.
 // FP64 keeps the checker's branch-by-branch arithmetic.
 template <typename F>
-__device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>& V, const DevSdom<F>& S,
+__device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>& V, int32_t pbeg,
                                              const dev::Event& ev, const uint4& b0, const uint4& b1,
                                              int hit, Lane<F>& L, uint32_t& fate) {
     const bool intr = hit < 0;
-    const DevPlane<F>& P = V.planes[intr ? S.pbeg : hit];
+    const DevPlane<F>& P = V.planes[intr ? pbeg : hit];
     int kind = intr ? -1 : P.kind;
     if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < P.spec_thr ? MCG_SPEC : MCG_DIFF;
     const bool diff = kind == MCG_DIFF;
@@ -899,14 +899,19 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
 template <typename F, int kDefer>
 __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V, const Sink& sink,
                                            Lane<F>& L, WalkOf<F, kDefer>& W, bool& walking, uint32_t& fate,
-                                           F& dmin_out, dev::V3<F>& apos) {
+                                           F& dmin_out, dev::V3<F>& apos, int32_t& pbeg_out,
+                                           int32_t& pore_out) {
     walking = false;
     const DevSdom<F>& S = V.sdoms[L.sdom];
     const F vel = V.vel[L.mode];
     // the cell's fields the flight tests, read once and together (a lone
-    // lane would otherwise wait for each read behind its own branch)
-    const int32_t s_box = S.box, s_pore = S.pore;
+    // lane would otherwise wait for each read behind its own branch); the
+    // collision gets the face base and the pore from here rather than read
+    // them again behind the tally's LDS atomics
+    const int32_t s_box = S.box, s_pore = S.pore, s_pbeg = S.pbeg;
     const F s_eps = S.eps;
+    pbeg_out = s_pbeg;
+    pore_out = s_pore;
     // advect: nearest plane the bundle moves towards (subdomain.cpp:161-192)
     F dmin = L.scat_next;
     int hit = -1;
@@ -929,7 +934,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
             const F p[3] = {L.pos.x, L.pos.y, L.pos.z}, u[3] = {L.dir.x, L.dir.y, L.dir.z};
             F lo[3], hi[3];
             int32_t flo[3], fhi[3];
-            const int32_t pbeg = S.pbeg;
+            const int32_t pbeg = s_pbeg;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 lo[a] = S.bmin[a];
@@ -954,7 +959,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
             // slot past the cell's last face (every cell has at least four)
             // reads the cell's first face and is then ignored, so that the
             // face reads issue together rather than behind a branch each.
-            const int32_t pbeg = S.pbeg, pend = S.pend;
+            const int32_t pbeg = s_pbeg, pend = S.pend;
 #pragma unroll
             for (int i = 0; i < kMaxFaces; ++i) {
                 const int b = pbeg + i;
@@ -973,7 +978,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
             }
             // faces beyond the first kMaxFaces (prisms and pyramids with
             // large bases) are tested again after the move
-            for (int b = S.pbeg + kMaxFaces; b < S.pend; ++b) {
+            for (int b = s_pbeg + kMaxFaces; b < S.pend; ++b) {
                 const DevPlane<F>& P = V.planes[b];
                 const F nd = dev::dot3(P.n, L.dir);
                 const F d = __builtin_fmaf(P.n[2], L.pos.z, __builtin_fmaf(P.n[1], L.pos.y,
@@ -1002,7 +1007,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 if (u[a] == F(0)) continue;
-                const int b = S.pbeg + (u[a] > F(0) ? S.fmax[a] : S.fmin[a]);
+                const int b = s_pbeg + (u[a] > F(0) ? S.fmax[a] : S.fmin[a]);
                 const DevPlane<F>& P = V.planes[b];
                 const F t = -(P.c + dev::dot3(P.n, L.pos)) / dev::dot3(P.n, L.dir);
                 if (t < dmin || (t == dmin && hit >= 0 && b < hit)) {
@@ -1011,7 +1016,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
                 }
             }
         } else {
-            for (int b = S.pbeg; b < S.pend; ++b) {
+            for (int b = s_pbeg; b < S.pend; ++b) {
                 const DevPlane<F>& P = V.planes[b];
                 const F nd = dev::dot3(P.n, L.dir);
                 if (nd >= F(0)) continue;
@@ -1061,7 +1066,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
             outside = false;
 #pragma unroll
             for (int i = 0; i < kMaxFaces; ++i) outside |= __builtin_fmaf(dmin, rate[i], dist[i]) < -s_eps;
-            for (int b = S.pbeg + kMaxFaces; b < S.pend; ++b) {
+            for (int b = s_pbeg + kMaxFaces; b < S.pend; ++b) {
                 const DevPlane<F>& P = V.planes[b];
                 outside |= dev::dot3(P.n, L.pos) + P.c < -s_eps;
             }
@@ -1107,8 +1112,7 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
 // bundle's subdomain is still the flight's.  Sets fate when the bundle ends.
 template <typename F>
 __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>& V, int hit, Lane<F>& L,
-                                             uint32_t& fate) {
-    const DevSdom<F>& S = V.sdoms[L.sdom];
+                                             uint32_t& fate, int32_t pbeg, int32_t pore) {
     {
         // the event's Philox blocks serve every branch below; they are drawn
         // once, outside the branches, so that a wavefront whose lanes take
@@ -1117,14 +1121,14 @@ __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>&
         const uint4 b0 = ev.block(0), b1 = low_block<F>(ev, 1);
         if constexpr (sizeof(F) == 4) {
             if (hit != -3) {
-                scatter_fp32<F>(K, V, S, ev, b0, b1, hit, L, fate);
+                scatter_fp32<F>(K, V, pbeg, ev, b0, b1, hit, L, fate);
             } else {
                 L.nbdry++;
-                pore_scatter<F>(V.pores[S.pore], ev, b0, b1, L);
+                pore_scatter<F>(V.pores[pore], ev, b0, b1, L);
             }
         } else {
             const bool intr = hit == -1;
-            const DevPlane<F>& P = V.planes[hit >= 0 ? hit : S.pbeg];
+            const DevPlane<F>& P = V.planes[hit >= 0 ? hit : pbeg];
             int kind = P.kind;
             if (kind == MCG_PARTIAL) kind = b1.w < P.spec_thr ? MCG_SPEC : MCG_DIFF;
             if (intr || (hit >= 0 && kind == MCG_DIFF)) {
@@ -1184,7 +1188,7 @@ __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>&
                 }
             } else {
                 L.nbdry++;
-                pore_scatter<F>(V.pores[S.pore], ev, b0, b1, L);
+                pore_scatter<F>(V.pores[pore], ev, b0, b1, L);
             }
         }
         if (fate == 0xFFFFFFFFu) {
@@ -1192,6 +1196,14 @@ __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>&
             else if (L.nstep >= K.maxloop) fate = MCG_MAXLOOP;
         }
     }
+}
+
+// The collision with the cell's face base and pore read here.
+template <typename F>
+__device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>& V, int hit, Lane<F>& L,
+                                             uint32_t& fate) {
+    const DevSdom<F>& S = V.sdoms[L.sdom];
+    collide_step<F>(K, V, hit, L, fate, S.pbeg, S.pore);
 }
 
 // One scatter-step of the bundle in L (problem.cpp:401-435): the flight and
@@ -1205,8 +1217,9 @@ __device__ __forceinline__ uint32_t scatter_step(const KParams<F>& K, const View
     uint32_t fate = 0xFFFFFFFFu;
     F dmin;
     dev::V3<F> apos;
-    const int hit = flight_step<F, kDefer>(K, V, sink, L, W, walking, fate, dmin, apos);
-    if (fate == 0xFFFFFFFFu) collide_step<F>(K, V, hit, L, fate);
+    int32_t pbeg, pore;
+    const int hit = flight_step<F, kDefer>(K, V, sink, L, W, walking, fate, dmin, apos, pbeg, pore);
+    if (fate == 0xFFFFFFFFu) collide_step<F>(K, V, hit, L, fate, pbeg, pore);
     if (L.id == K.trace_id && int64_t(L.nstep) <= K.trace_cap) {
         double* r = K.trace + (int64_t(L.nstep) - 1) * MCG_TRACE_WIDTH;
         r[0] = double(L.nstep);
@@ -1646,7 +1659,8 @@ __device__ __forceinline__ int ev_flight(const KParams<F>& K, const View<F>& V, 
     bool walking = false;
     F dmin;
     dev::V3<F> apos;
-    const int hit = flight_step<F, 0>(K, V, sink, L, W, walking, fate, dmin, apos);
+    int32_t pbeg, pore;
+    const int hit = flight_step<F, 0>(K, V, sink, L, W, walking, fate, dmin, apos, pbeg, pore);
     if (fate == 0xFFFFFFFFu && hit >= 0 && V.planes[hit].kind == MCG_ISOT) collide_step<F>(K, V, hit, L, fate);
     return hit;
 }
