@@ -844,7 +844,14 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
                                              int hit, Lane<F>& L, uint32_t& fate) {
     const bool intr = hit < 0;
     const DevPlane<F>& P = V.planes[intr ? pbeg : hit];
-    int kind = intr ? -1 : P.kind;
+    // the face's kind and its rotation (what a diffuse face, the common
+    // case, needs) read together; a specular or periodic face reads its own
+    // matrix afterwards
+    const int32_t pkind = P.kind;
+    F m[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m[i] = P.rot[i];
+    int kind = intr ? -1 : pkind;
     if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < P.spec_thr ? MCG_SPEC : MCG_DIFF;
     const bool diff = kind == MCG_DIFF;
     // direction: isotropic cos(theta) = r, or Lambertian sin^2(theta) = |r|
@@ -857,8 +864,12 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
     dev::Real<F>::sincos_pi(q, sq, cq);
     const dev::V3<F> v{sinT * cq, sinT * sq, cosT};
     const bool turn = diff || kind == MCG_SPEC || kind == MCG_PERI;
-    const F* M = kind == MCG_SPEC ? P.refl : kind == MCG_PERI ? P.prot : P.rot;
-    const dev::V3<F> out = dev::matvec(M, diff ? v : L.dir);
+    if (kind == MCG_SPEC || kind == MCG_PERI) {
+        const F* M = kind == MCG_SPEC ? P.refl : P.prot;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) m[i] = M[i];
+    }
+    const dev::V3<F> out = dev::matvec(m, diff ? v : L.dir);
     if (intr || turn) L.dir = dev::normalized(intr ? v : out);
     L.nscat += (intr || diff) ? 1u : 0u;
     L.nbdry += intr ? 0u : 1u;
