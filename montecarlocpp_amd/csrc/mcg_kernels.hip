@@ -1014,17 +1014,25 @@ __device__ __forceinline__ int flight_step(const KParams<F>& K, const View<F>& V
             // equal parameters keep the lower face index, as the loop in face
             // order does, and a face only displaces the intrinsic event when
             // strictly nearer
+            // The three faces' numbers, then their planes, are read
+            // together, and the parameters computed before they are compared
+            // in axis order (an axis the bundle does not move along is
+            // skipped by the compare, not by a branch).
             const F u[3] = {L.dir.x, L.dir.y, L.dir.z};
+            int32_t fb[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) fb[a] = s_pbeg + (u[a] > F(0) ? S.fmax[a] : S.fmin[a]);
+            F tt[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                if (u[a] == F(0)) continue;
-                const int b = s_pbeg + (u[a] > F(0) ? S.fmax[a] : S.fmin[a]);
-                const DevPlane<F>& P = V.planes[b];
-                const F t = -(P.c + dev::dot3(P.n, L.pos)) / dev::dot3(P.n, L.dir);
-                if (t < dmin || (t == dmin && hit >= 0 && b < hit)) {
-                    dmin = t;
-                    hit = b;
-                }
+                const DevPlane<F>& P = V.planes[fb[a]];
+                tt[a] = -(P.c + dev::dot3(P.n, L.pos)) / dev::dot3(P.n, L.dir);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const bool take = u[a] != F(0) && (tt[a] < dmin || (tt[a] == dmin && hit >= 0 && fb[a] < hit));
+                dmin = take ? tt[a] : dmin;
+                hit = take ? fb[a] : hit;
             }
         } else {
             for (int b = s_pbeg; b < S.pend; ++b) {
