@@ -668,6 +668,9 @@ int build_domain(const mcg_domain_spec* spec, HostScene& s) {
             P.ibeg = int32_t(s.inter_list.size());
             P.icnt = int32_t(inter_pairs[i].size());
             s.inter_list.insert(s.inter_list.end(), inter_pairs[i].begin(), inter_pairs[i].end());
+            // a face with one partner carries it in the plane record, so a
+            // crossing reads no partner list
+            if (P.icnt == 1) P.target = inter_pairs[i].front();
         }
     }
     // pores (cylinder along z, cut out of one subdomain)

@@ -885,7 +885,7 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
     } else if (kind == MCG_INTER) {
         int target = -1;
         if (P.icnt == 1) {
-            target = K.inter_list[P.ibeg];
+            target = P.target;
         } else {
             for (int c = 0; c < P.icnt; ++c) {
                 const int cand = K.inter_list[P.ibeg + c];
@@ -1190,7 +1190,7 @@ __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>&
                     default: {       // MCG_INTER, boundary.cpp:349-359
                         int target = -1;
                         if (P.icnt == 1) {
-                            target = K.inter_list[P.ibeg];
+                            target = P.target;
                         } else {
                             for (int q = 0; q < P.icnt; ++q) {
                                 const int cand = K.inter_list[P.ibeg + q];

@@ -49,7 +49,7 @@ struct DevPlane {
     F pt[3];          // periodic map translation
     int32_t kind;     // mcg_bdry_kind
     int32_t sdom;     // owning subdomain
-    int32_t target;   // Peri: subdomain of the paired face
+    int32_t target;   // Peri: subdomain of the paired face; Inter with one partner: that partner
     int32_t ibeg;     // Inter: first candidate in the pair list
     int32_t icnt;     // Inter: number of candidates
     uint32_t spec_thr;
