@@ -612,7 +612,8 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
             W.last[k] = ei[k] + 1;
             W.dcol[k] = -S.stride[k];
         }
-        W.q[k] = sizeof(F) == 8 ? dc : F(1) / dc;
+        if constexpr (sizeof(F) == 8) W.q[k] = dc;
+        else W.q[k] = __builtin_amdgcn_rcpf(dc);   // FP32 flight: the hardware reciprocal
         W.pk[k] = crossing_param<F>(W.nxt[k], bc[k], W.q[k]);
         if constexpr (sizeof(F) == 4) {
             // FP32 flight steps the parameter by |1/dc| per crossing and
@@ -638,7 +639,13 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
 template <typename F>
 __device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Event& ev, const uint4& b0,
                                              const uint4& b1, Lane<F>& L) {
-    const dev::V3<F> n = dev::normalized(dev::V3<F>{(L.pos.x - Q.cx) / Q.r, (L.pos.y - Q.cy) / Q.r, F(0)});
+    // FP64 divides by the radius as the checker does; FP32 flight
+    // normalises the offset from the axis directly (the same direction)
+    dev::V3<F> n;
+    if constexpr (sizeof(F) == 8)
+        n = dev::normalized(dev::V3<F>{(L.pos.x - Q.cx) / Q.r, (L.pos.y - Q.cy) / Q.r, F(0)});
+    else
+        n = dev::normalized(dev::V3<F>{L.pos.x - Q.cx, L.pos.y - Q.cy, F(0)});
     int kind = Q.kind;
     if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
     if (kind == MCG_SPEC) {
