@@ -69,3 +69,23 @@ def test_graph_batch_replays_equal_eager(precision):
             assert np.all(rownorm_rel(fg[k], fe2[k]) < 1e-10)
     for s in sc:
         s.close()
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_FLIGHT])
+def test_batch_of_one_scene_repeated(precision):
+    """One scene listed several times in a batch (a stream of solves of one
+    geometry, each with its own seed, and one seed twice): every entry gives
+    what its own mcg_solve gives, and the repeated seed gives the same field."""
+    with engine.Scene(materials.si_like(300.0), scenes.bar(4e-7, 1e-7, 1e-7, 30, t_hot=10.0, t_cold=-10.0)) as sc:
+        seeds = [5, 6, 7, 5]
+        probs = [scenes.problem(40_000, seed=s, precision=precision) for s in seeds]
+        print("repeated scene", len(probs), "solves", flush=True)
+        fb, sb = engine.batch_solve([sc] * len(probs), probs)
+        for k, p in enumerate(probs):
+            f1, s1, _ = sc.solve(p)
+            assert sb[k]["steps"] == s1["steps"], k
+            assert sb[k]["bundles"] == s1["bundles"] == 40_000
+            assert np.all(rownorm_rel(fb[k], f1) < 1e-10), k
+        assert sb[0]["steps"] == sb[3]["steps"]
+        assert np.all(rownorm_rel(fb[0], fb[3]) < 1e-10)
+        assert sb[0]["steps"] != sb[1]["steps"]
