@@ -33,7 +33,7 @@ checker) and "engines" (the history kernel against the event engine on the
 bar, the porous film and the octet truss).
 --headline-only skips the secondary objects and the CPU leg.
 
-    python bench.py --only slab|porous_film|octet|batched|serving|engines
+    python bench.py --only bar_streamed|slab|porous_film|octet|batched|serving|engines
 
 runs one secondary object alone (for profiling it).
 
@@ -124,11 +124,13 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE, scaled by the copy-probe calibration)")
     ap.add_argument("--quick", action="store_true",
                     help="with --only engines: one launch shape per engine, one timed solve each")
-    ap.add_argument("--only", choices=("slab", "porous_film", "octet", "batched", "serving", "engines"),
+    ap.add_argument("--only", choices=("bar_streamed", "slab", "porous_film", "octet", "batched", "serving",
+                                       "engines"),
                     help="run one secondary workload alone and print its JSON object")
     args = ap.parse_args()
     if args.only:
-        run = {"slab": lambda: bench_slab(cpu=not args.no_cpu),
+        run = {"bar_streamed": lambda: bar_streamed_alone(args.steps),
+               "slab": lambda: bench_slab(cpu=not args.no_cpu),
                "porous_film": lambda: bench_porous(cpu=not args.no_cpu),
                "octet": lambda: bench_octet(cpu=not args.no_cpu),
                "batched": lambda: bench_batched(cpu=not args.no_cpu),
@@ -321,6 +323,15 @@ def bench_bar_streamed(sc, seed, k, fields):
             "vs_one_at_a_time_rel_diff": max(rel(a, b) for a, b in zip(fs, fields)),
             "note": "the headline's K solves (same seeds) as one mcg_batch_solve; wall clock "
                     "including the batch's set-up and result copies"}
+
+
+def bar_streamed_alone(k):
+    """bench_bar_streamed by itself (for profiling it): the K one-at-a-time
+    solves it is compared with, then the K solves as one batch launch."""
+    mat, dom = workload()
+    with engine.Scene(mat, dom) as sc:
+        fields = [sc.solve(scenes.problem(NBUNDLES, seed=1 + i, precision=abi.FP32_FLIGHT))[0] for i in range(k)]
+        return bench_bar_streamed(sc, 1, k, fields)
 
 
 def bench_porous(cpu=True, nbundles=100_000_000, cpu_sample=1_000_000):
