@@ -152,8 +152,8 @@ def main():
     if args.headline_only:
         args.no_cpu = True
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the solve shards by "
-              "bundle range across GPUs, which this build does not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); a solve would shard by "
+              "bundle range across GPUs, which is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     mat, dom = workload()
