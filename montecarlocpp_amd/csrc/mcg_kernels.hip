@@ -202,16 +202,12 @@ __device__ __forceinline__ uint4 low_block(const dev::Event& ev, uint32_t j) {
     else return make_uint4(0u, 0u, 0u, 0u);
 }
 
-// The partial-specularity word.  FP64 takes word 3 of block 1, as the
-// checker does.  FP32 flight takes word 3 of block 0 instead: a boundary
-// event leaves it unused (it is the intrinsic scatter's mode draw) and it is
-// independent of the direction words 0 and 1, so the decision has the same
-// distribution without a second Philox block per partial face; FP32 and FP64
-// runs of one stream then part at the first partial-face hit.
+// The partial-specularity word (word 3 of block 1): FP64 has the block at
+// hand, FP32 evaluates it here, on a partial face only.
 template <typename F>
-__device__ __forceinline__ uint32_t spec_word(const uint4& b0, const uint4& b1) {
+__device__ __forceinline__ uint32_t spec_word(const dev::Event& ev, const uint4& b1) {
     if constexpr (sizeof(F) == 8) return b1.w;
-    else return b0.w;
+    else return ev.block(1).w;
 }
 
 // Free path: l = v tau (-ln(1 - u)), redrawn while l < the smallest normal
@@ -641,7 +637,8 @@ __device__ __forceinline__ bool accumulate(const DevSdom<F>& S, const Sink& sink
 // points away from the axis (into the material); the reflection and the
 // Lambertian rotation are built from it as for a planar face.
 template <typename F>
-__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const uint4& b0, const uint4& b1, Lane<F>& L) {
+__device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const dev::Event& ev, const uint4& b0,
+                                             const uint4& b1, Lane<F>& L) {
     // FP64 divides by the radius as the checker does; FP32 flight
     // normalises the offset from the axis directly (the same direction)
     dev::V3<F> n;
@@ -650,7 +647,7 @@ __device__ __forceinline__ void pore_scatter(const DevPore<F>& Q, const uint4& b
     else
         n = dev::normalized(dev::V3<F>{L.pos.x - Q.cx, L.pos.y - Q.cy, F(0)});
     int kind = Q.kind;
-    if (kind == MCG_PARTIAL) kind = spec_word<F>(b0, b1) < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
+    if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < Q.spec_thr ? MCG_SPEC : MCG_DIFF;
     if (kind == MCG_SPEC) {
         const F nv[3] = {n.x, n.y, n.z};
         F r[9];
@@ -862,7 +859,7 @@ __device__ __forceinline__ void scatter_fp32(const KParams<F>& K, const View<F>&
 #pragma unroll
     for (int i = 0; i < 9; ++i) m[i] = P.rot[i];
     int kind = intr ? -1 : pkind;
-    if (kind == MCG_PARTIAL) kind = spec_word<F>(b0, b1) < P.spec_thr ? MCG_SPEC : MCG_DIFF;
+    if (kind == MCG_PARTIAL) kind = spec_word<F>(ev, b1) < P.spec_thr ? MCG_SPEC : MCG_DIFF;
     const bool diff = kind == MCG_DIFF;
     // direction: isotropic cos(theta) = r, or Lambertian sin^2(theta) = |r|
     const F r = dev::Real<F>::uniform(b0.x, b1.x) * F(2) + F(-1);
@@ -1153,7 +1150,7 @@ __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>&
                 scatter_fp32<F>(K, V, pbeg, ev, b0, b1, hit, L, fate);
             } else {
                 L.nbdry++;
-                pore_scatter<F>(V.pores[pore], b0, b1, L);
+                pore_scatter<F>(V.pores[pore], ev, b0, b1, L);
             }
         } else {
             const bool intr = hit == -1;
@@ -1217,7 +1214,7 @@ __device__ __forceinline__ void collide_step(const KParams<F>& K, const View<F>&
                 }
             } else {
                 L.nbdry++;
-                pore_scatter<F>(V.pores[pore], b0, b1, L);
+                pore_scatter<F>(V.pores[pore], ev, b0, b1, L);
             }
         }
         if (fate == 0xFFFFFFFFu) {

@@ -19,8 +19,7 @@ namespace mcg {
 // scatter after loop iteration i).  A double uniform in slot w takes its high
 // word from w and its low word from w + 4, the same word of the next block;
 // the FP32 flight path uses the high word alone, so a scatter costs it one
-// Philox block (its partial-specularity test takes W_MODE, which a boundary
-// event leaves unused, rather than W_SPEC of block 1).  Free-path
+// Philox block (block 1 only for a partial-specularity test).  Free-path
 // redraws use adjacent pairs (W_REDRAW + 2k, + 2k + 1).
 enum : int {
     W_DIR1 = 0,

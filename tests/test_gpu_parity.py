@@ -218,8 +218,7 @@ def test_fp32_production_within_3_sigma(si300):
 
 
 def test_fp32_same_streams_close(si300):
-    """FP32 flight follows the same random decisions as FP64 (all but the
-    partial-specularity test, which has no partial face here).  On the
+    """FP32 flight follows the same random decisions as FP64.  On the
     specular-wall slab (no diffuse bounces to amplify rounding) almost every
     bundle keeps identical counts and the tallies agree to 1e-4; diffuse-wall
     geometries decorrelate the two precisions after tens of bounces, so there
